@@ -1,0 +1,79 @@
+// engine.h -- internal state of libcassgpu (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cass_gpu.h"
+
+struct cass_box_state {
+  cass_params p{};
+  int32_t cap_mols = 0, cap_sites = 0;
+  int32_t nmols = 0, nsites = 0;
+  std::vector<int32_t> mol_species, mol_first, live;  // host mirrors
+  std::vector<double> xyz_h;        // host mirror of the coordinates (3*nsites)
+  // device: sites (SoA, FP64 master) and molecules
+  double *x = nullptr, *y = nullptr, *z = nullptr;
+  int32_t *site_spa = nullptr;      // species-atom index of each site
+  int32_t *d_first = nullptr, *d_species = nullptr, *d_live = nullptr;
+  double4 *comd = nullptr;          // (xcom, ycom, zcom, max_dcom)
+  std::vector<double4> comd_h;      // host mirror of comd
+  // Ewald (k order of Ewald_Reciprocal_Lattice_Vector_Setup)
+  int32_t nvecs = 0;
+  std::vector<double> hx_h, hy_h, hz_h, cn_h;
+  std::vector<int32_t> kn_h;        // (nx, ny, nz) per k, 3 ints
+  std::vector<int4> rows_h;         // rows of consecutive nx at fixed (ny, nz): (ny, nz, nx0, count)
+  double *hx = nullptr, *hy = nullptr, *hz = nullptr, *cn = nullptr;
+  double *cs = nullptr, *sn = nullptr;            // box structure factor
+  double *cmol = nullptr, *smol = nullptr;        // [cap_mols][nvecs]
+  double *cs_new = nullptr, *sn_new = nullptr;    // staged box sums of a move
+  double *cm_new = nullptr, *sm_new = nullptr;    // staged molecule sums of a move
+  int4 *rows = nullptr;
+  int32_t nrows = 0;
+  int32_t staged_mol = -1, staged_flag = -1;
+  bool ewald_ready = false;
+  double e_recip = 0.0;             // last full / committed total reciprocal energy
+  double e_recip_staged = 0.0;
+};
+
+struct cass_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t prec = CASS_PREC_FP64;
+  std::string err;
+  // topology (host copies + device)
+  int32_t ntypes = 0, nspecies = 0, natoms_total = 0;
+  std::vector<double> eps, sig, mie_n, mie_m;
+  std::vector<int32_t> sp_natoms, sp_atom_off, sp_pair_off, sp_coul_cbmc;
+  std::vector<int32_t> atom_type;
+  std::vector<double> atom_charge, atom_mass, vdw_scale, q_scale;
+  std::vector<int32_t> bonded;
+  double *d_eps = nullptr, *d_sig = nullptr, *d_mn = nullptr, *d_mm = nullptr;
+  int32_t *d_atom_type = nullptr;
+  double *d_atom_charge = nullptr, *d_atom_mass = nullptr;
+  int32_t *d_sp_natoms = nullptr, *d_sp_atom_off = nullptr, *d_sp_coul_cbmc = nullptr;
+  std::vector<cass_box_state> boxes;
+  // rigid Widom templates per species: site offsets from the COM and the
+  // intramolecular energy of the template (cass_widom_set_template)
+  std::vector<std::vector<double>> widom_ref;
+  std::vector<double> widom_intra;
+  int32_t max_boxes = 0;
+  // scratch (grown on demand)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *pinned = nullptr;
+  size_t pinned_bytes = 0;
+  void *io = nullptr;               // device input/output staging of the small calls
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets the last batched kernel
+  double last_kernel_ms = 0.0;
+  size_t io_bytes = 0;
+};
+
+// host helpers shared between translation units
+int cass_fail(cass_engine *e, int code, const std::string &msg);
+int cass_check_hip(cass_engine *e, hipError_t st, const char *what);
+void *cass_scratch(cass_engine *e, size_t bytes);
+void *cass_pinned(cass_engine *e, size_t bytes);
+void *cass_io(cass_engine *e, size_t bytes);

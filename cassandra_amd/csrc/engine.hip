@@ -1,0 +1,958 @@
+// engine.hip -- host side of the C ABI (include/cass_gpu.h): device state,
+// uploads and kernel launches.  One HIP stream per engine; every entry point
+// that returns values synchronises on it.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+
+using namespace cass;
+
+#define HIP_TRY(e, call)                                          \
+  do {                                                            \
+    hipError_t st_ = (call);                                      \
+    if (st_ != hipSuccess) return cass_check_hip((e), st_, #call); \
+  } while (0)
+
+int cass_fail(cass_engine *e, int code, const std::string &msg) {
+  if (e) e->err = msg;
+  return code;
+}
+
+int cass_check_hip(cass_engine *e, hipError_t st, const char *what) {
+  if (st == hipSuccess) return CASS_OK;
+  return cass_fail(e, st == hipErrorOutOfMemory ? CASS_ERR_NOMEM : CASS_ERR_HIP,
+                   std::string(what) + ": " + hipGetErrorString(st));
+}
+
+void *cass_scratch(cass_engine *e, size_t bytes) {
+  if (bytes <= e->scratch_bytes) return e->scratch;
+  if (e->scratch) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(e->scratch);
+  }
+  e->scratch = nullptr;
+  e->scratch_bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 20);
+  if (hipMalloc(&e->scratch, want) != hipSuccess) return nullptr;
+  e->scratch_bytes = want;
+  return e->scratch;
+}
+
+void *cass_pinned(cass_engine *e, size_t bytes) {
+  if (bytes <= e->pinned_bytes) return e->pinned;
+  if (e->pinned) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipHostFree(e->pinned);
+  }
+  e->pinned = nullptr;
+  e->pinned_bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 16);
+  if (hipHostMalloc(&e->pinned, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+  e->pinned_bytes = want;
+  return e->pinned;
+}
+
+void *cass_io(cass_engine *e, size_t bytes) {
+  if (bytes <= e->io_bytes) return e->io;
+  if (e->io) {
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(e->io);
+  }
+  e->io = nullptr;
+  e->io_bytes = 0;
+  size_t want = std::max(bytes, (size_t)1 << 16);
+  if (hipMalloc(&e->io, want) != hipSuccess) return nullptr;
+  e->io_bytes = want;
+  return e->io;
+}
+
+template <typename X>
+static hipError_t dalloc(X **p, size_t n) {
+  return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(X));
+}
+
+template <typename X>
+static hipError_t upload(cass_engine *e, X *dst, const X *src, size_t n) {
+  if (n == 0) return hipSuccess;
+  return hipMemcpyAsync(dst, src, n * sizeof(X), hipMemcpyHostToDevice, e->stream);
+}
+
+static BoxDev box_view(const cass_engine *e, const cass_box_state &b) {
+  BoxDev v;
+  v.x = b.x;
+  v.y = b.y;
+  v.z = b.z;
+  v.first = b.d_first;
+  v.species = b.d_species;
+  v.live = b.d_live;
+  v.comd = b.comd;
+  v.nmols = b.nmols;
+  v.cubic = b.p.box_shape == CASS_BOX_CUBIC;
+  v.vdw_minimum = b.p.vdw_sum_style == CASS_SUM_MINIMUM;
+  for (int k = 0; k < 3; ++k) {
+    v.L[k] = b.p.cell[4 * k];
+    v.hL[k] = b.p.hlength[k];
+  }
+  for (int k = 0; k < 9; ++k) {
+    v.Li[k] = b.p.length_inv[k];
+    v.Lc[k] = b.p.cell[k];
+  }
+  v.rcut_max = b.p.rcut_max;
+  v.rcut_cbmc = b.p.rcut_cbmc;
+  (void)e;
+  return v;
+}
+
+static TopoDev topo_view(const cass_engine *e) {
+  TopoDev t;
+  t.eps = e->d_eps;
+  t.sig = e->d_sig;
+  t.mn = e->d_mn;
+  t.mm = e->d_mm;
+  t.atom_type = e->d_atom_type;
+  t.atom_charge = e->d_atom_charge;
+  t.atom_mass = e->d_atom_mass;
+  t.sp_natoms = e->d_sp_natoms;
+  t.sp_atom_off = e->d_sp_atom_off;
+  t.sp_coul_cbmc = e->d_sp_coul_cbmc;
+  t.ntypes = e->ntypes;
+  return t;
+}
+
+static EwaldDev ewald_view(const cass_box_state &b, const int *row_k0) {
+  EwaldDev w;
+  w.nvecs = b.nvecs;
+  w.nrows = b.nrows;
+  w.hx = b.hx;
+  w.hy = b.hy;
+  w.hz = b.hz;
+  w.cn = b.cn;
+  w.cs = b.cs;
+  w.sn = b.sn;
+  w.cmol = b.cmol;
+  w.smol = b.smol;
+  w.cs_new = b.cs_new;
+  w.sn_new = b.sn_new;
+  w.cm_new = b.cm_new;
+  w.sm_new = b.sm_new;
+  w.rows = b.rows;
+  w.row_k0 = row_k0;
+  return w;
+}
+
+static bool species_charged(const cass_engine *e, int s) {
+  for (int a = 0; a < e->sp_natoms[s]; ++a)
+    if (e->atom_charge[e->sp_atom_off[s] + a] != 0.0) return true;
+  return false;
+}
+
+// ------------------------------------------------------------------ setup
+extern "C" int cass_engine_create(int32_t device, const cass_topology *t, int32_t max_boxes,
+                                  cass_engine **out) {
+  if (!t || !out || max_boxes <= 0) return CASS_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return CASS_ERR_NODEVICE;
+  if (device < 0 || device >= ndev) return CASS_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return CASS_ERR_HIP;
+  cass_engine *e = new cass_engine();
+  e->device = device;
+  e->max_boxes = max_boxes;
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+    delete e;
+    return CASS_ERR_HIP;
+  }
+  e->ntypes = t->ntypes;
+  e->nspecies = t->nspecies;
+  const size_t nt2 = (size_t)t->ntypes * t->ntypes;
+  e->eps.assign(t->eps, t->eps + nt2);
+  e->sig.assign(t->sig, t->sig + nt2);
+  e->mie_n = t->mie_n ? std::vector<double>(t->mie_n, t->mie_n + nt2) : std::vector<double>(nt2, 0.0);
+  e->mie_m = t->mie_m ? std::vector<double>(t->mie_m, t->mie_m + nt2) : std::vector<double>(nt2, 0.0);
+  e->sp_natoms.assign(t->species_natoms, t->species_natoms + t->nspecies);
+  e->sp_atom_off.assign(t->species_atom_offset, t->species_atom_offset + t->nspecies);
+  e->sp_pair_off.assign(t->species_pair_offset, t->species_pair_offset + t->nspecies);
+  e->sp_coul_cbmc.assign(t->species_coul_cbmc, t->species_coul_cbmc + t->nspecies);
+  int na = 0, np = 0;
+  for (int s = 0; s < t->nspecies; ++s) {
+    if (t->species_natoms[s] > kMaxSites) {
+      delete e;
+      return CASS_ERR_ARG;
+    }
+    na = std::max(na, t->species_atom_offset[s] + t->species_natoms[s]);
+    np = std::max(np, t->species_pair_offset[s] + t->species_natoms[s] * t->species_natoms[s]);
+  }
+  e->natoms_total = na;
+  e->atom_type.assign(t->atom_type, t->atom_type + na);
+  e->atom_charge.assign(t->atom_charge, t->atom_charge + na);
+  e->atom_mass.assign(t->atom_mass, t->atom_mass + na);
+  e->vdw_scale.assign(t->vdw_intra_scale, t->vdw_intra_scale + np);
+  e->q_scale.assign(t->charge_intra_scale, t->charge_intra_scale + np);
+  e->bonded.assign(t->bonded, t->bonded + np);
+  HIP_TRY(e, dalloc(&e->d_eps, nt2));
+  HIP_TRY(e, dalloc(&e->d_sig, nt2));
+  HIP_TRY(e, dalloc(&e->d_mn, nt2));
+  HIP_TRY(e, dalloc(&e->d_mm, nt2));
+  HIP_TRY(e, dalloc(&e->d_atom_type, na));
+  HIP_TRY(e, dalloc(&e->d_atom_charge, na));
+  HIP_TRY(e, dalloc(&e->d_atom_mass, na));
+  HIP_TRY(e, dalloc(&e->d_sp_natoms, t->nspecies));
+  HIP_TRY(e, dalloc(&e->d_sp_atom_off, t->nspecies));
+  HIP_TRY(e, dalloc(&e->d_sp_coul_cbmc, t->nspecies));
+  HIP_TRY(e, upload(e, e->d_eps, e->eps.data(), nt2));
+  HIP_TRY(e, upload(e, e->d_sig, e->sig.data(), nt2));
+  HIP_TRY(e, upload(e, e->d_mn, e->mie_n.data(), nt2));
+  HIP_TRY(e, upload(e, e->d_mm, e->mie_m.data(), nt2));
+  HIP_TRY(e, upload(e, e->d_atom_type, e->atom_type.data(), na));
+  HIP_TRY(e, upload(e, e->d_atom_charge, e->atom_charge.data(), na));
+  HIP_TRY(e, upload(e, e->d_atom_mass, e->atom_mass.data(), na));
+  HIP_TRY(e, upload(e, e->d_sp_natoms, e->sp_natoms.data(), t->nspecies));
+  HIP_TRY(e, upload(e, e->d_sp_atom_off, e->sp_atom_off.data(), t->nspecies));
+  HIP_TRY(e, upload(e, e->d_sp_coul_cbmc, e->sp_coul_cbmc.data(), t->nspecies));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  e->boxes.reserve(max_boxes);
+  *out = e;
+  return CASS_OK;
+}
+
+static void free_box(cass_box_state &b) {
+  void *ps[] = {b.x, b.y, b.z, b.site_spa, b.d_first, b.d_species, b.d_live, b.comd, b.hx, b.hy, b.hz,
+                b.cn, b.cs, b.sn, b.cmol, b.smol, b.cs_new, b.sn_new, b.cm_new, b.sm_new, b.rows};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+}
+
+extern "C" int cass_engine_destroy(cass_engine *e) {
+  if (!e) return CASS_OK;
+  (void)hipSetDevice(e->device);
+  (void)hipStreamSynchronize(e->stream);
+  for (auto &b : e->boxes) free_box(b);
+  void *ps[] = {e->d_eps, e->d_sig, e->d_mn, e->d_mm, e->d_atom_type, e->d_atom_charge, e->d_atom_mass,
+                e->d_sp_natoms, e->d_sp_atom_off, e->d_sp_coul_cbmc, e->scratch, e->io};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  if (e->pinned) (void)hipHostFree(e->pinned);
+  (void)hipEventDestroy(e->ev0);
+  (void)hipEventDestroy(e->ev1);
+  (void)hipStreamDestroy(e->stream);
+  delete e;
+  return CASS_OK;
+}
+
+extern "C" int cass_set_precision(cass_engine *e, int32_t prec) {
+  if (!e || (prec != CASS_PREC_FP64 && prec != CASS_PREC_MIXED)) return CASS_ERR_ARG;
+  e->prec = prec;
+  return CASS_OK;
+}
+
+extern "C" const char *cass_last_error(cass_engine *e) { return e ? e->err.c_str() : "no engine"; }
+
+extern "C" int cass_last_kernel_ms(cass_engine *e, double *ms) {
+  if (!e || !ms) return CASS_ERR_ARG;
+  *ms = e->last_kernel_ms;
+  return CASS_OK;
+}
+
+extern "C" int cass_box_create(cass_engine *e, const cass_params *p, int32_t max_mols, int32_t *box) {
+  if (!e || !p || !box || max_mols <= 0) return CASS_ERR_ARG;
+  if ((int)e->boxes.size() >= e->max_boxes) return cass_fail(e, CASS_ERR_ARG, "max_boxes reached");
+  const double half = 0.5 * std::min({p->basis_length[0], p->basis_length[1], p->basis_length[2]});
+  if (p->vdw_style != CASS_VDW_NONE && p->vdw_sum_style != CASS_SUM_MINIMUM && p->rcut_vdw > half * (1 + 1e-12))
+    return cass_fail(e, CASS_ERR_CUTOFF, "vdW cutoff exceeds half the box");
+  HIP_TRY(e, hipSetDevice(e->device));
+  e->boxes.emplace_back();
+  cass_box_state &b = e->boxes.back();
+  b.p = *p;
+  b.cap_mols = max_mols;
+  int maxn = *std::max_element(e->sp_natoms.begin(), e->sp_natoms.end());
+  b.cap_sites = max_mols * maxn;
+  HIP_TRY(e, dalloc(&b.x, b.cap_sites));
+  HIP_TRY(e, dalloc(&b.y, b.cap_sites));
+  HIP_TRY(e, dalloc(&b.z, b.cap_sites));
+  HIP_TRY(e, dalloc(&b.site_spa, b.cap_sites));
+  HIP_TRY(e, dalloc(&b.d_first, max_mols));
+  HIP_TRY(e, dalloc(&b.d_species, max_mols));
+  HIP_TRY(e, dalloc(&b.d_live, max_mols));
+  HIP_TRY(e, dalloc(&b.comd, max_mols));
+  *box = (int32_t)e->boxes.size() - 1;
+  return CASS_OK;
+}
+
+static int box_of(cass_engine *e, int32_t box, cass_box_state **out) {
+  if (!e) return CASS_ERR_ARG;
+  if (box < 0 || box >= (int)e->boxes.size()) return cass_fail(e, CASS_ERR_ARG, "bad box index");
+  *out = &e->boxes[box];
+  return hipSetDevice(e->device) == hipSuccess ? CASS_OK : CASS_ERR_HIP;
+}
+
+static double4 host_com(const cass_engine *e, int s, const double *xyz) {
+  const int o = e->sp_atom_off[s], n = e->sp_natoms[s];
+  double tm = 0, cx = 0, cy = 0, cz = 0;
+  for (int a = 0; a < n; ++a) {
+    const double m = e->atom_mass[o + a];
+    tm = tm + m;
+    cx = cx + m * xyz[3 * a];
+    cy = cy + m * xyz[3 * a + 1];
+    cz = cz + m * xyz[3 * a + 2];
+  }
+  cx = cx / tm;
+  cy = cy / tm;
+  cz = cz / tm;
+  double dmax = 0;
+  for (int a = 0; a < n; ++a) {
+    const double dx = xyz[3 * a] - cx, dy = xyz[3 * a + 1] - cy, dz = xyz[3 * a + 2] - cz;
+    dmax = std::max(dmax, dx * dx + dy * dy + dz * dz);
+  }
+  return make_double4(cx, cy, cz, std::sqrt(dmax));
+}
+
+extern "C" int cass_box_upload(cass_engine *e, int32_t box, int32_t nmols, const int32_t *mol_species,
+                               const int32_t *live, const double *xyz, const double *com,
+                               const double *max_dcom) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (nmols < 0 || nmols > b->cap_mols) return cass_fail(e, CASS_ERR_ARG, "nmols exceeds box capacity");
+  b->nmols = nmols;
+  b->mol_species.assign(mol_species, mol_species + nmols);
+  b->live.assign(live, live + nmols);
+  b->mol_first.resize(nmols);
+  int ns = 0;
+  for (int m = 0; m < nmols; ++m) {
+    if (mol_species[m] < 0 || mol_species[m] >= e->nspecies) return cass_fail(e, CASS_ERR_ARG, "bad species");
+    b->mol_first[m] = ns;
+    ns += e->sp_natoms[mol_species[m]];
+  }
+  b->nsites = ns;
+  b->xyz_h.assign(xyz, xyz + 3 * (size_t)ns);
+  std::vector<double> xs(ns), ys(ns), zs(ns);
+  std::vector<int32_t> spa(ns);
+  std::vector<double4> cd(nmols);
+  for (int m = 0; m < nmols; ++m) {
+    const int s = mol_species[m], f = b->mol_first[m];
+    for (int a = 0; a < e->sp_natoms[s]; ++a) {
+      xs[f + a] = xyz[3 * (f + a)];
+      ys[f + a] = xyz[3 * (f + a) + 1];
+      zs[f + a] = xyz[3 * (f + a) + 2];
+      spa[f + a] = e->sp_atom_off[s] + a;
+    }
+    double4 c = host_com(e, s, xyz + 3 * f);
+    if (com) c = make_double4(com[3 * m], com[3 * m + 1], com[3 * m + 2], c.w);
+    if (max_dcom) c.w = max_dcom[m];
+    cd[m] = c;
+  }
+  HIP_TRY(e, upload(e, b->x, xs.data(), ns));
+  HIP_TRY(e, upload(e, b->y, ys.data(), ns));
+  HIP_TRY(e, upload(e, b->z, zs.data(), ns));
+  HIP_TRY(e, upload(e, b->site_spa, spa.data(), ns));
+  HIP_TRY(e, upload(e, b->d_first, b->mol_first.data(), nmols));
+  HIP_TRY(e, upload(e, b->d_species, b->mol_species.data(), nmols));
+  HIP_TRY(e, upload(e, b->d_live, b->live.data(), nmols));
+  HIP_TRY(e, upload(e, b->comd, cd.data(), nmols));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  b->comd_h = cd;
+  b->ewald_ready = false;
+  return CASS_OK;
+}
+
+extern "C" int cass_mol_update(cass_engine *e, int32_t box, int32_t mol, int32_t live, const double *xyz,
+                               const double *com, const double *max_dcom) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (mol < 0 || mol >= b->nmols) return cass_fail(e, CASS_ERR_ARG, "bad molecule index");
+  const int s = b->mol_species[mol], n = e->sp_natoms[s], f = b->mol_first[mol];
+  // one pinned staging buffer: x[n], y[n], z[n], comd, live
+  double *h = (double *)cass_pinned(e, sizeof(double) * (3 * n + 4) + 16);
+  if (!h) return cass_fail(e, CASS_ERR_NOMEM, "pinned staging");
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  if (xyz) {
+    std::memcpy(b->xyz_h.data() + 3 * (size_t)f, xyz, sizeof(double) * 3 * n);
+    for (int a = 0; a < n; ++a) {
+      h[a] = xyz[3 * a];
+      h[n + a] = xyz[3 * a + 1];
+      h[2 * n + a] = xyz[3 * a + 2];
+    }
+    double4 c = host_com(e, s, xyz);
+    if (com) c = make_double4(com[0], com[1], com[2], c.w);
+    if (max_dcom) c.w = *max_dcom;
+    std::memcpy(h + 3 * n, &c, sizeof(double4));
+    b->comd_h[mol] = c;
+    HIP_TRY(e, upload(e, b->x + f, h, n));
+    HIP_TRY(e, upload(e, b->y + f, h + n, n));
+    HIP_TRY(e, upload(e, b->z + f, h + 2 * n, n));
+    HIP_TRY(e, upload(e, b->comd + mol, (const double4 *)(h + 3 * n), 1));
+  }
+  b->live[mol] = live;
+  int32_t *hl = (int32_t *)(h + 3 * n + 4);
+  *hl = live;
+  HIP_TRY(e, upload(e, b->d_live + mol, hl, 1));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  return CASS_OK;
+}
+
+extern "C" int cass_box_download(cass_engine *e, int32_t box, double *xyz) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  std::vector<double> xs(b->nsites), ys(b->nsites), zs(b->nsites);
+  HIP_TRY(e, hipMemcpyAsync(xs.data(), b->x, sizeof(double) * b->nsites, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(ys.data(), b->y, sizeof(double) * b->nsites, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(zs.data(), b->z, sizeof(double) * b->nsites, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (int i = 0; i < b->nsites; ++i) {
+    xyz[3 * i] = xs[i];
+    xyz[3 * i + 1] = ys[i];
+    xyz[3 * i + 2] = zs[i];
+  }
+  return CASS_OK;
+}
+
+// ------------------------------------------------------- trial energies
+// Launches k_trials + k_finalize_trials; results (E_vdw, E_qq, overlap)
+// land in dev_out[2*t], dev_ov[t] (device) -- the caller copies back.
+static int launch_trials(cass_engine *e, cass_box_state &b, int species, int ntrial, const double *d_txyz,
+                         int cbmc, int exclude, double *dev_out, int *dev_ov) {
+  const int ni = e->sp_natoms[species];
+  const int blk = kMaxBlock;
+  const int nblk = std::max(1, (b.nmols + blk - 1) / blk);
+  const size_t np = (size_t)ntrial * nblk;
+  char *s = (char *)cass_scratch(e, np * (2 * sizeof(double) + sizeof(int)) + 64);
+  if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
+  double *part = (double *)s;
+  int *part_ov = (int *)(s + np * 2 * sizeof(double));
+  BoxDev bv = box_view(e, b);
+  TopoDev tv = topo_view(e);
+  PairConst<double> cd = make_pair_const<double>(b.p, e->ntypes);
+  dim3 grid(nblk, ntrial);
+  if (e->prec == CASS_PREC_MIXED) {
+    PairConst<float> cf = make_pair_const<float>(b.p, e->ntypes);
+    hipLaunchKernelGGL(k_trials<float>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, species, ni, d_txyz,
+                       exclude, cbmc, part, part_ov);
+  } else {
+    hipLaunchKernelGGL(k_trials<double>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cd, species, ni, d_txyz,
+                       exclude, cbmc, part, part_ov);
+  }
+  HIP_TRY(e, hipGetLastError());
+  hipLaunchKernelGGL(k_finalize_trials, dim3((ntrial + 255) / 256), dim3(256), 0, e->stream, part, part_ov,
+                     nblk, ntrial, dev_out, dev_ov);
+  HIP_TRY(e, hipGetLastError());
+  return CASS_OK;
+}
+
+extern "C" int cass_trial_energies(cass_engine *e, int32_t box, int32_t species, int32_t ntrial,
+                                   const double *trial_xyz, int32_t cbmc_flag, int32_t exclude_mol,
+                                   double *nrg, int32_t *overlap) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (species < 0 || species >= e->nspecies || ntrial <= 0 || !trial_xyz)
+    return cass_fail(e, CASS_ERR_ARG, "bad trial arguments");
+  const int ni = e->sp_natoms[species];
+  const size_t nxyz = (size_t)ntrial * ni * 3;
+  // io buffers live in their own allocation (scratch holds partials)
+  double *d_io = (double *)cass_io(e, sizeof(double) * (nxyz + 3 * (size_t)ntrial) + 16);
+  if (!d_io) return cass_fail(e, CASS_ERR_NOMEM, "io buffer");
+  double *d_out = d_io + nxyz;
+  int *d_ov = (int *)(d_out + 2 * (size_t)ntrial);
+  HIP_TRY(e, hipMemcpyAsync(d_io, trial_xyz, sizeof(double) * nxyz, hipMemcpyHostToDevice, e->stream));
+  rc = launch_trials(e, *b, species, ntrial, d_io, cbmc_flag, exclude_mol, d_out, d_ov);
+  if (rc) return rc;
+  std::vector<double> out(2 * (size_t)ntrial);
+  std::vector<int> ov(ntrial);
+  HIP_TRY(e, hipMemcpyAsync(out.data(), d_out, sizeof(double) * 2 * ntrial, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(ov.data(), d_ov, sizeof(int) * ntrial, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (int t = 0; t < ntrial; ++t) {
+    nrg[t] = out[2 * t] + out[2 * t + 1];
+    overlap[t] = ov[t];
+  }
+  return CASS_OK;
+}
+
+// Energies of up to two placements of molecule `mol` in one launch (used by
+// cass_mol_inter_energy and by the NVT chain for old + new positions).
+int cass_mol_pair_trials(cass_engine *e, int32_t box, int32_t mol, int32_t cbmc, int32_t ntrial,
+                         const double *xyz2, double *ev, double *eq, int32_t *ov) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  const int s = b->mol_species[mol], n = e->sp_natoms[s];
+  const size_t nx = (size_t)ntrial * n * 3;
+  char *h = (char *)cass_pinned(e, sizeof(double) * (nx + 3 * ntrial) + 64);
+  double *d_io = (double *)cass_io(e, sizeof(double) * (nx + 3 * ntrial) + 64);
+  if (!h || !d_io) return cass_fail(e, CASS_ERR_NOMEM, "staging");
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  std::memcpy(h, xyz2, sizeof(double) * nx);
+  HIP_TRY(e, hipMemcpyAsync(d_io, h, sizeof(double) * nx, hipMemcpyHostToDevice, e->stream));
+  double *d_out = d_io + nx;
+  int *d_ov = (int *)(d_out + 2 * ntrial);
+  rc = launch_trials(e, *b, s, ntrial, d_io, cbmc, mol, d_out, d_ov);
+  if (rc) return rc;
+  double *ho = (double *)(h + sizeof(double) * nx);
+  HIP_TRY(e, hipMemcpyAsync(ho, d_out, sizeof(double) * 2 * ntrial + sizeof(int) * ntrial,
+                            hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  const int *hov = (const int *)(ho + 2 * ntrial);
+  for (int t = 0; t < ntrial; ++t) {
+    ev[t] = ho[2 * t];
+    eq[t] = ho[2 * t + 1];
+    ov[t] = hov[t];
+  }
+  return CASS_OK;
+}
+
+const double *cass_box_xyz(cass_engine *e, int32_t box, int32_t mol) {
+  cass_box_state &b = e->boxes[box];
+  return b.xyz_h.data() + 3 * (size_t)b.mol_first[mol];
+}
+
+extern "C" int cass_mol_inter_energy(cass_engine *e, int32_t box, int32_t mol, int32_t cbmc_flag,
+                                     const double *xyz, const double *com, const double *max_dcom,
+                                     double *e_vdw, double *e_qq, int32_t *overlap) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (mol < 0 || mol >= b->nmols) return cass_fail(e, CASS_ERR_ARG, "bad molecule index");
+  (void)com;
+  (void)max_dcom;
+  const double *src = xyz ? xyz : cass_box_xyz(e, box, mol);
+  return cass_mol_pair_trials(e, box, mol, cbmc_flag, 1, src, e_vdw, e_qq, overlap);
+}
+
+// ----------------------------------------------------------- total energy
+int cass_intra_host(const cass_engine *e, const cass_params &p, int s, int cbmc, const double *xyz,
+                    double *ev, double *eq, double *eqi);  // host.cpp
+
+extern "C" int cass_total_energy(cass_engine *e, int32_t box, int32_t intra_flag, cass_energy *out,
+                                 int32_t *overlap) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  std::memset(out, 0, sizeof(*out));
+  *overlap = 0;
+  // intramolecular nonbonded terms on the host (n_m^2 pairs per molecule)
+  const std::vector<double> &xyz = b->xyz_h;
+  if (intra_flag) {
+    for (int s = 0; s < e->nspecies; ++s) {
+      double vi = 0, vq = 0, vqi = 0;
+      for (int m = 0; m < b->nmols; ++m) {
+        if (b->mol_species[m] != s || !b->live[m]) continue;
+        double a, q, qi;
+        if (cass_intra_host(e, b->p, s, 0, xyz.data() + 3 * b->mol_first[m], &a, &q, &qi)) {
+          *overlap = 1;
+          return CASS_OK;
+        }
+        vi = vi + q + a;
+        vq = vq + q;
+        vqi = vqi + qi;
+        out->intra_vdw += a;
+      }
+      out->intra += vi;
+      out->intra_q += vq;
+      out->inter_q += vqi;
+    }
+  }
+  // intermolecular pairs on the device
+  const int blk = kMaxBlock;
+  const int nblk = std::max(1, (b->nmols + blk - 1) / blk);
+  const size_t np = (size_t)nblk * std::max(1, b->nmols);
+  char *s = (char *)cass_scratch(e, np * (2 * sizeof(double) + sizeof(int)) + 256);
+  if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
+  double *part = (double *)s;
+  int *part_ov = (int *)(s + np * 2 * sizeof(double));
+  if (b->nmols > 0) {
+    BoxDev bv = box_view(e, *b);
+    TopoDev tv = topo_view(e);
+    PairConst<double> cd = make_pair_const<double>(b->p, e->ntypes);
+    dim3 grid(nblk, b->nmols);
+    if (e->prec == CASS_PREC_MIXED) {
+      PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
+      hipLaunchKernelGGL(k_total_pairs<float>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, part, part_ov);
+    } else {
+      hipLaunchKernelGGL(k_total_pairs<double>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cd, part, part_ov);
+    }
+    HIP_TRY(e, hipGetLastError());
+    std::vector<double> hp(2 * np);
+    std::vector<int> ho(np);
+    HIP_TRY(e, hipMemcpyAsync(hp.data(), part, sizeof(double) * 2 * np, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(ho.data(), part_ov, sizeof(int) * np, hipMemcpyDeviceToHost, e->stream));
+    HIP_TRY(e, hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < np; ++i) {
+      out->inter_vdw += hp[2 * i];
+      out->inter_q += hp[2 * i + 1];
+      if (ho[i]) *overlap = 1;
+    }
+    if (*overlap) return CASS_OK;
+  }
+  out->inter = out->inter_q + out->inter_vdw;
+  const cass_params &p = b->p;
+  if (p.charge_style != CASS_Q_NONE) {
+    if (p.charge_style == CASS_Q_EWALD) {
+      if (!b->ewald_ready) {
+        int32_t nv;
+        if ((rc = cass_ewald_setup(e, box, &nv))) return rc;
+      }
+      double er;
+      if ((rc = cass_ewald_full(e, box, &er))) return rc;
+      out->reciprocal = er;
+      out->inter += er;
+    }
+    double qq = 0.0;
+    for (int s2 = 0; s2 < e->nspecies; ++s2)
+      for (int m = 0; m < b->nmols; ++m) {
+        if (b->mol_species[m] != s2 || !b->live[m]) continue;
+        for (int a = 0; a < e->sp_natoms[s2]; ++a) {
+          const double q = e->atom_charge[e->sp_atom_off[s2] + a];
+          qq = qq + q * q;
+        }
+      }
+    if (p.charge_style == CASS_Q_EWALD) out->self = -qq * kChargeFactor * p.alpha_ewald / kRootPi;
+    else if (p.charge_style == CASS_Q_DSF)
+      out->self = -(qq * (p.alpha_dsf / kRootPi + p.dsf_factor1 / 2.0)) * kChargeFactor;
+    out->inter += out->self;
+  }
+  if (p.vdw_sum_style == CASS_SUM_CUT_TAIL) {
+    std::vector<double> nb(e->ntypes, 0.0);
+    for (int m = 0; m < b->nmols; ++m) {
+      if (!b->live[m]) continue;
+      const int s2 = b->mol_species[m];
+      for (int a = 0; a < e->sp_natoms[s2]; ++a) {
+        const int t = e->atom_type[e->sp_atom_off[s2] + a];
+        if (t >= 0) nb[t] += 1.0;
+      }
+    }
+    double lrc;
+    if ((rc = cass_lrc(e, box, nb.data(), &lrc))) return rc;
+    out->lrc = lrc;
+    out->inter += lrc;
+  }
+  out->total = out->intra + out->inter;
+  return CASS_OK;
+}
+
+// ------------------------------------------------------------------ Ewald
+extern "C" int cass_ewald_setup(cass_engine *e, int32_t box, int32_t *nvecs) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  const cass_params &p = b->p;
+  if (p.charge_style != CASS_Q_EWALD) return cass_fail(e, CASS_ERR_STATE, "box is not Ewald");
+  // Ewald_Reciprocal_Lattice_Vector_Setup (energy_routines.f90:2170-2323)
+  const double cv = 1.0 / (4.0 * p.alpha_ewald * p.alpha_ewald);
+  const double hc2 = p.h_ewald_cut * p.h_ewald_cut;
+  std::vector<double> hx, hy, hz, cn;
+  std::vector<int32_t> kn;
+  auto add = [&](int nx, int ny, int nz, double a, double bb, double c) {
+    const double h2 = a * a + bb * bb + c * c;
+    if (h2 < hc2) {
+      hx.push_back(a);
+      hy.push_back(bb);
+      hz.push_back(c);
+      cn.push_back((nx == 0 ? kTwoPi : 2.0 * kTwoPi) / p.volume * std::exp(-h2 * cv) / h2);
+      kn.push_back(nx);
+      kn.push_back(ny);
+      kn.push_back(nz);
+    }
+  };
+  if (p.box_shape != CASS_BOX_CUBIC) {
+    const double *Li = p.length_inv;
+    for (int nz = -20; nz <= 20; ++nz)
+      for (int ny = -20; ny <= 20; ++ny)
+        for (int nx = 0; nx <= 20; ++nx) {
+          if (nx == 0 && ny == 0 && nz == 0) continue;
+          const double X = nx, Y = ny, Z = nz;
+          add(nx, ny, nz, kTwoPi * (Li[0] * X + Li[1] * Y + Li[2] * Z),
+              kTwoPi * (Li[3] * X + Li[4] * Y + Li[5] * Z), kTwoPi * (Li[6] * X + Li[7] * Y + Li[8] * Z));
+        }
+  } else {
+    const int kz = (int)(p.h_ewald_cut * p.basis_length[2] / kTwoPi) + 1;
+    const int ky = (int)(p.h_ewald_cut * p.basis_length[1] / kTwoPi) + 1;
+    const int kx = (int)(p.h_ewald_cut * p.basis_length[0] / kTwoPi) + 1;
+    int count = 1;
+    for (int nz = -kz; nz <= kz; ++nz)
+      for (int ny = -ky; ny <= ky; ++ny)
+        for (int nx = 0; nx <= kx; ++nx) {
+          if (count > 100000) return cass_fail(e, CASS_ERR_KVEC, "Total number of k vectors exceeded");
+          if (nx == 0 && ny == 0 && nz == 0) continue;
+          const size_t before = hx.size();
+          add(nx, ny, nz, kTwoPi * (double)nx / p.basis_length[0], kTwoPi * (double)ny / p.basis_length[1],
+              kTwoPi * (double)nz / p.basis_length[2]);
+          count += (int)(hx.size() - before);
+        }
+  }
+  const int K = (int)hx.size();
+  if (K > 100000) return cass_fail(e, CASS_ERR_KVEC, "Total number of k vectors exceeded");
+  // rows of consecutive nx at fixed (ny, nz) for the phase recurrence
+  std::vector<int4> rows;
+  std::vector<int32_t> k0;
+  for (int k = 0; k < K; ++k) {
+    const int nx = kn[3 * k], ny = kn[3 * k + 1], nz = kn[3 * k + 2];
+    if (!rows.empty()) {
+      int4 &r = rows.back();
+      if (r.x == ny && r.y == nz && r.z + r.w == nx) {
+        r.w += 1;
+        continue;
+      }
+    }
+    rows.push_back(make_int4(ny, nz, nx, 1));
+    k0.push_back(k);
+  }
+  // (re)allocate the k-space arrays of this box
+  void *old[] = {b->hx, b->hy, b->hz, b->cn, b->cs, b->sn, b->cmol, b->smol, b->cs_new, b->sn_new,
+                 b->cm_new, b->sm_new, b->rows};
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (void *q : old)
+    if (q) (void)hipFree(q);
+  HIP_TRY(e, dalloc(&b->hx, K));
+  HIP_TRY(e, dalloc(&b->hy, K));
+  HIP_TRY(e, dalloc(&b->hz, K));
+  HIP_TRY(e, dalloc(&b->cn, K));
+  HIP_TRY(e, dalloc(&b->cs, K));
+  HIP_TRY(e, dalloc(&b->sn, K));
+  HIP_TRY(e, dalloc(&b->cs_new, K));
+  HIP_TRY(e, dalloc(&b->sn_new, K));
+  HIP_TRY(e, dalloc(&b->cm_new, K));
+  HIP_TRY(e, dalloc(&b->sm_new, K));
+  HIP_TRY(e, dalloc(&b->cmol, (size_t)K * b->cap_mols));
+  HIP_TRY(e, dalloc(&b->smol, (size_t)K * b->cap_mols));
+  // rows and their first-k table share one allocation
+  HIP_TRY(e, dalloc(&b->rows, rows.size() + (k0.size() + 3) / 4 + 1));
+  HIP_TRY(e, upload(e, b->hx, hx.data(), K));
+  HIP_TRY(e, upload(e, b->hy, hy.data(), K));
+  HIP_TRY(e, upload(e, b->hz, hz.data(), K));
+  HIP_TRY(e, upload(e, b->cn, cn.data(), K));
+  HIP_TRY(e, upload(e, b->rows, rows.data(), rows.size()));
+  HIP_TRY(e, upload(e, (int32_t *)(b->rows + rows.size()), k0.data(), k0.size()));
+  HIP_TRY(e, hipMemsetAsync(b->cs, 0, sizeof(double) * std::max(K, 1), e->stream));
+  HIP_TRY(e, hipMemsetAsync(b->sn, 0, sizeof(double) * std::max(K, 1), e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  b->nvecs = K;
+  b->nrows = (int)rows.size();
+  b->hx_h = hx;
+  b->hy_h = hy;
+  b->hz_h = hz;
+  b->cn_h = cn;
+  b->kn_h = kn;
+  b->rows_h = rows;
+  b->ewald_ready = true;
+  b->staged_mol = -1;
+  *nvecs = K;
+  return CASS_OK;
+}
+
+static const int *row_k0_of(const cass_box_state &b) { return (const int *)(b.rows + b.nrows); }
+
+extern "C" int cass_ewald_full(cass_engine *e, int32_t box, double *e_recip) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (!b->ewald_ready) return cass_fail(e, CASS_ERR_STATE, "call cass_ewald_setup first");
+  const int K = b->nvecs;
+  BoxDev bv = box_view(e, *b);
+  TopoDev tv = topo_view(e);
+  EwaldDev ew = ewald_view(*b, row_k0_of(*b));
+  if (b->nmols > 0 && K > 0) {
+    hipLaunchKernelGGL(k_ewald_mol, dim3((K + 255) / 256, b->nmols), dim3(256), 0, e->stream, bv, tv, ew);
+    HIP_TRY(e, hipGetLastError());
+  }
+  const int nb = std::max(1, (K + 255) / 256);
+  double *part = (double *)cass_scratch(e, sizeof(double) * (nb + 1));
+  if (!part) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
+  hipLaunchKernelGGL(k_ewald_box, dim3(nb), dim3(256), 0, e->stream, bv, ew, part);
+  HIP_TRY(e, hipGetLastError());
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, e->stream, part, (size_t)nb, 1, 0, part + nb);
+  HIP_TRY(e, hipGetLastError());
+  double E;
+  HIP_TRY(e, hipMemcpyAsync(&E, part + nb, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  b->e_recip = E * kChargeFactor;
+  b->staged_mol = -1;
+  *e_recip = b->e_recip;
+  return CASS_OK;
+}
+
+extern "C" int cass_ewald_move(cass_engine *e, int32_t box, int32_t mol, int32_t move_flag, const double *xyz,
+                               double *e_recip_total, double *de_recip) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (!b->ewald_ready) return cass_fail(e, CASS_ERR_STATE, "call cass_ewald_setup first");
+  if (mol < 0 || mol >= b->cap_mols) return cass_fail(e, CASS_ERR_ARG, "bad molecule index");
+  if (move_flag < CASS_MOVE_TRANSLATION || move_flag > CASS_MOVE_DELETION) return CASS_ERR_ARG;
+  if (move_flag != CASS_MOVE_DELETION && !xyz) return CASS_ERR_ARG;
+  const int s = mol < b->nmols ? b->mol_species[mol] : 0;
+  const int n = e->sp_natoms[s];
+  const int K = b->nvecs;
+  const int nb = std::max(1, (K + 255) / 256);
+  char *h = (char *)cass_pinned(e, sizeof(double) * (3 * n + 4));
+  if (!h) return cass_fail(e, CASS_ERR_NOMEM, "pinned staging");
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  double *part = (double *)cass_scratch(e, sizeof(double) * (2 * nb + 3 * n + 8));
+  if (!part) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
+  double *d_xyz = part + 2 * nb + 4;
+  if (xyz) {
+    std::memcpy(h, xyz, sizeof(double) * 3 * n);
+    HIP_TRY(e, hipMemcpyAsync(d_xyz, h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, e->stream));
+  }
+  TopoDev tv = topo_view(e);
+  EwaldDev ew = ewald_view(*b, row_k0_of(*b));
+  hipLaunchKernelGGL(k_ewald_move, dim3(nb), dim3(256), 0, e->stream, tv, ew, mol, s, move_flag, n, d_xyz, part);
+  HIP_TRY(e, hipGetLastError());
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, e->stream, part, (size_t)nb, 2, 0, part + 2 * nb);
+  hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, e->stream, part, (size_t)nb, 2, 1, part + 2 * nb + 1);
+  HIP_TRY(e, hipGetLastError());
+  double *ho = (double *)h;
+  HIP_TRY(e, hipMemcpyAsync(ho, part + 2 * nb, sizeof(double) * 2, hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  b->e_recip_staged = ho[0] * kChargeFactor;
+  b->staged_mol = mol;
+  b->staged_flag = move_flag;
+  if (e_recip_total) *e_recip_total = b->e_recip_staged;
+  if (de_recip) *de_recip = ho[1] * kChargeFactor;
+  return CASS_OK;
+}
+
+extern "C" int cass_ewald_commit(cass_engine *e, int32_t box) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (b->staged_mol < 0) return cass_fail(e, CASS_ERR_STATE, "no staged Ewald move");
+  EwaldDev ew = ewald_view(*b, row_k0_of(*b));
+  hipLaunchKernelGGL(k_ewald_commit, dim3(std::max(1, (b->nvecs + 255) / 256)), dim3(256), 0, e->stream, ew,
+                     b->staged_mol);
+  HIP_TRY(e, hipGetLastError());
+  b->e_recip = b->e_recip_staged;
+  b->staged_mol = -1;
+  return CASS_OK;
+}
+
+extern "C" int cass_ewald_revert(cass_engine *e, int32_t box) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  b->staged_mol = -1;  // the staged sums are simply dropped
+  return CASS_OK;
+}
+
+extern "C" int cass_ewald_widom(cass_engine *e, int32_t box, int32_t species, int32_t ntrial,
+                                const double *trial_xyz, double *e_recip) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (!b->ewald_ready) return cass_fail(e, CASS_ERR_STATE, "call cass_ewald_setup first");
+  if (species < 0 || species >= e->nspecies || ntrial <= 0) return CASS_ERR_ARG;
+  const int n = e->sp_natoms[species];
+  const int K = b->nvecs;
+  const int nbk = std::max(1, std::min(8, (K + 255) / 256));
+  const size_t nxyz = (size_t)ntrial * n * 3;
+  double *d_io = nullptr;
+  HIP_TRY(e, hipMallocAsync((void **)&d_io, sizeof(double) * (nxyz + (size_t)ntrial * nbk) + 16, e->stream));
+  double *part = d_io + nxyz;
+  HIP_TRY(e, hipMemcpyAsync(d_io, trial_xyz, sizeof(double) * nxyz, hipMemcpyHostToDevice, e->stream));
+  TopoDev tv = topo_view(e);
+  EwaldDev ew = ewald_view(*b, row_k0_of(*b));
+  hipLaunchKernelGGL(k_ewald_widom, dim3(nbk, ntrial), dim3(256), 0, e->stream, tv, ew, species, n, d_io, part);
+  HIP_TRY(e, hipGetLastError());
+  std::vector<double> hp((size_t)ntrial * nbk);
+  HIP_TRY(e, hipMemcpyAsync(hp.data(), part, sizeof(double) * hp.size(), hipMemcpyDeviceToHost, e->stream));
+  HIP_TRY(e, hipFreeAsync(d_io, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (int t = 0; t < ntrial; ++t) {
+    double s = 0.0;
+    for (int k = 0; k < nbk; ++k) s += hp[(size_t)t * nbk + k];
+    e_recip[t] = s * kChargeFactor;
+  }
+  return CASS_OK;
+}
+
+// ----------------------------------------------------------------- Widom
+int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs &w);  // host.cpp
+
+static int run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed,
+                     const double *place, double *widom_var, double *eout, cass_widom_result *res) {
+  cass_box_state *b;
+  int rc = box_of(e, box, &b);
+  if (rc) return rc;
+  if (species < 0 || species >= e->nspecies || n_ins <= 0) return CASS_ERR_ARG;
+  if (e->sp_natoms[species] > kMaxWidomSites) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+  WidomArgs w;
+  if ((rc = cass_widom_prepare(e, *b, species, w))) return rc;
+  w.n_ins = n_ins;
+  w.seed = seed;
+  const int nblk = (int)((n_ins + kWidomBlock - 1) / kWidomBlock);
+  const size_t ni = e->sp_natoms[species];
+  size_t bytes = sizeof(double) * nblk + sizeof(int64_t) * nblk + 64;
+  double *d_place = nullptr, *d_var = nullptr, *d_eout = nullptr;
+  char *s = (char *)cass_scratch(e, bytes);
+  if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
+  double *part = (double *)s;
+  int64_t *part_ov = (int64_t *)(s + sizeof(double) * nblk);
+  double *d_sum = (double *)(s + (sizeof(double) + sizeof(int64_t)) * nblk);
+  int64_t *d_nov = (int64_t *)(d_sum + 1);
+  if (place) {
+    HIP_TRY(e, hipMallocAsync((void **)&d_place, sizeof(double) * n_ins * ni * 3, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_place, place, sizeof(double) * n_ins * ni * 3, hipMemcpyHostToDevice, e->stream));
+  }
+  if (widom_var) HIP_TRY(e, hipMallocAsync((void **)&d_var, sizeof(double) * n_ins, e->stream));
+  if (eout) HIP_TRY(e, hipMallocAsync((void **)&d_eout, sizeof(double) * 4 * n_ins, e->stream));
+  w.place = d_place;
+  BoxDev bv = box_view(e, *b);
+  TopoDev tv = topo_view(e);
+  EwaldDev ew = ewald_view(*b, b->rows ? row_k0_of(*b) : nullptr);
+  if (!b->ewald_ready) {
+    ew.nvecs = 0;
+    ew.nrows = 0;
+  }
+  PairConst<double> cd = make_pair_const<double>(b->p, e->ntypes);
+  HIP_TRY(e, hipEventRecord(e->ev0, e->stream));
+  if (e->prec == CASS_PREC_MIXED) {
+    PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
+    hipLaunchKernelGGL(k_widom<float>, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cf,
+                       part, part_ov, d_var, d_eout);
+  } else {
+    hipLaunchKernelGGL(k_widom<double>, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cd,
+                       part, part_ov, d_var, d_eout);
+  }
+  HIP_TRY(e, hipGetLastError());
+  HIP_TRY(e, hipEventRecord(e->ev1, e->stream));
+  hipLaunchKernelGGL(k_sum_widom, dim3(1), dim3(256), 0, e->stream, part, part_ov, nblk, d_sum, d_nov);
+  HIP_TRY(e, hipGetLastError());
+  double hs[2];
+  HIP_TRY(e, hipMemcpyAsync(hs, d_sum, sizeof(double) * 2, hipMemcpyDeviceToHost, e->stream));
+  if (widom_var) HIP_TRY(e, hipMemcpyAsync(widom_var, d_var, sizeof(double) * n_ins, hipMemcpyDeviceToHost, e->stream));
+  if (eout) HIP_TRY(e, hipMemcpyAsync(eout, d_eout, sizeof(double) * 4 * n_ins, hipMemcpyDeviceToHost, e->stream));
+  if (d_place) HIP_TRY(e, hipFreeAsync(d_place, e->stream));
+  if (d_var) HIP_TRY(e, hipFreeAsync(d_var, e->stream));
+  if (d_eout) HIP_TRY(e, hipFreeAsync(d_eout, e->stream));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  float ms = 0.0f;
+  HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
+  e->last_kernel_ms = ms;
+  res->widom_sum = hs[0];
+  std::memcpy(&res->n_overlaps, &hs[1], sizeof(int64_t));
+  res->n_insertions = n_ins;
+  res->e_const = w.e_const;
+  return CASS_OK;
+}
+
+extern "C" int cass_widom_batch(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed,
+                                double *widom_var, cass_widom_result *res) {
+  if (!res) return CASS_ERR_ARG;
+  return run_widom(e, box, species, n_ins, seed, nullptr, widom_var, nullptr, res);
+}
+
+// Per-insertion body of Widom_Insert at explicit placements (test entry: the
+// same kernel as cass_widom_batch).  eout[4*i] = {E_inter, E_cbmc,
+// dE_recip, overlap}.
+extern "C" int cass_widom_placements(cass_engine *e, int32_t box, int32_t species, int64_t n,
+                                     const double *xyz, double *widom_var, double *eout,
+                                     cass_widom_result *res) {
+  if (!res || !xyz) return CASS_ERR_ARG;
+  return run_widom(e, box, species, n, 0, xyz, widom_var, eout, res);
+}

@@ -1,0 +1,336 @@
+// host.cpp -- host-side pieces of libcassgpu: derived box constants, the
+// reference's random-number stream and Metropolis test, intramolecular
+// energies, tail correction and self energy, the Widom constants, and the
+// NVT move loop that drives the engine (nvtmc_driver.f90, move_translate.f90,
+// move_rotate.f90).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "engine.h"
+#include "kernels.h"
+
+using namespace cass;
+
+// ------------------------------------------------------------- parameters
+extern "C" int cass_params_derive(cass_params *p) {
+  if (!p) return CASS_ERR_ARG;
+  // compute_cell_dimensions.f90:50-170
+  const double *L = p->cell;
+  auto LL = [&](int i, int j) { return L[(i - 1) + 3 * (j - 1)]; };
+  double *Li = p->length_inv;
+  auto LI = [&](int i, int j) -> double & { return Li[(i - 1) + 3 * (j - 1)]; };
+  for (int j = 1; j <= 3; ++j)
+    p->basis_length[j - 1] = std::sqrt(LL(1, j) * LL(1, j) + LL(2, j) * LL(2, j) + LL(3, j) * LL(3, j));
+  const double bxc1 = LL(2, 2) * LL(3, 3) - LL(3, 2) * LL(2, 3);
+  const double bxc2 = LL(3, 2) * LL(1, 3) - LL(1, 2) * LL(3, 3);
+  const double bxc3 = LL(1, 2) * LL(2, 3) - LL(2, 2) * LL(1, 3);
+  p->volume = std::fabs(LL(1, 1) * bxc1 + LL(2, 1) * bxc2 + LL(3, 1) * bxc3);
+  LI(1, 1) = LL(2, 2) * LL(3, 3) - LL(2, 3) * LL(3, 2);
+  LI(2, 1) = LL(3, 1) * LL(2, 3) - LL(2, 1) * LL(3, 3);
+  LI(3, 1) = LL(2, 1) * LL(3, 2) - LL(2, 2) * LL(3, 1);
+  LI(1, 2) = LL(3, 2) * LL(1, 3) - LL(1, 2) * LL(3, 3);
+  LI(2, 2) = LL(1, 1) * LL(3, 3) - LL(3, 1) * LL(1, 3);
+  LI(3, 2) = LL(3, 1) * LL(1, 2) - LL(1, 1) * LL(3, 2);
+  LI(1, 3) = LL(1, 2) * LL(2, 3) - LL(2, 2) * LL(1, 3);
+  LI(2, 3) = LL(2, 1) * LL(1, 3) - LL(1, 1) * LL(2, 3);
+  LI(3, 3) = LL(1, 1) * LL(2, 2) - LL(2, 1) * LL(1, 2);
+  const double det = LL(1, 1) * LI(1, 1) + LL(1, 2) * LI(2, 1) + LL(1, 3) * LI(3, 1);
+  if (det == 0.0) return CASS_ERR_ARG;
+  const double inv_det = 1.0 / det;
+  for (int k = 0; k < 9; ++k) Li[k] = Li[k] * inv_det;
+  for (int k = 0; k < 3; ++k) p->hlength[k] = 0.5 * p->basis_length[k];
+  const double face = p->volume / std::sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+  // input_routines.f90:425-630 and precalculate.f90:40-90
+  if (p->vdw_sum_style == CASS_SUM_MINIMUM) p->rcut_vdw = 0.5 * face;
+  if (p->vdw_sum_style == CASS_SUM_CUT_SWITCH) {
+    p->ron_switch_sq = p->ron_switch * p->ron_switch;
+    p->roff_switch_sq = p->roff_switch * p->roff_switch;
+    const double d = p->roff_switch_sq - p->ron_switch_sq;
+    p->switch_factor1 = 1.0 / std::pow(d, 3.0);
+    p->switch_factor2 = p->roff_switch_sq - 3.0 * p->ron_switch_sq;
+    p->rcut_vdw = p->roff_switch;
+    p->rcut_vdwsq = 0.0;
+  } else if (p->vdw_sum_style == CASS_SUM_CHARMM) {
+    p->rcut_vdw = 0.0;  // the CHARMM branch leaves rcut_vdw unset
+    p->rcut_vdwsq = 0.0;
+  } else {
+    p->rcut_vdwsq = p->rcut_vdw * p->rcut_vdw;
+  }
+  p->rcut3 = p->rcut_vdw * p->rcut_vdw * p->rcut_vdw;
+  p->rcut9 = p->rcut3 * p->rcut3 * p->rcut3;
+  if (p->charge_style == CASS_Q_MINIMUM) p->rcut_coul = 0.5 * face;
+  if (p->charge_style != CASS_Q_NONE) p->rcut_coulsq = p->rcut_coul * p->rcut_coul;
+  if (p->vdw_style != CASS_VDW_NONE && p->charge_style != CASS_Q_NONE)
+    p->rcut_max = std::max(p->rcut_vdw, p->rcut_coul);
+  else if (p->vdw_style != CASS_VDW_NONE)
+    p->rcut_max = p->rcut_vdw;
+  else
+    p->rcut_max = p->rcut_coul;
+  p->rcut_cbmcsq = p->rcut_cbmc * p->rcut_cbmc;
+  p->rcut_lowsq = p->rcut_low * p->rcut_low;
+  // input_routines.f90:693-716 (Ewald) and :729-757 (DSF, A&S erfc)
+  if (p->charge_style == CASS_Q_EWALD) {
+    const double ep = -std::log(p->ewald_tol);
+    p->alpha_ewald = std::sqrt(ep) / p->rcut_coul;
+    p->h_ewald_cut = 2.0 * ep / p->rcut_coul;
+  }
+  if (p->charge_style == CASS_Q_DSF) {
+    double a = p->alpha_dsf_in;
+    if (!(a > 0.0)) {
+      a = 0.425 - p->rcut_coul * 0.02;
+      if (a < 0.0) a = 3.3930702 / p->rcut_coul;
+    }
+    const double rc = p->rcut_coul;
+    p->alpha_dsf = a;
+    p->dsf_factor1 = erfc_as<double>(a * rc) / rc;
+    p->dsf_factor2 = p->dsf_factor1 / rc + 2.0 * a * std::exp(-a * a * rc * rc) / (kRootPi * rc);
+  }
+  p->beta = 1.0 / (0.8314472 * p->temperature);  // input_routines.f90:4090
+  return CASS_OK;
+}
+
+// ------------------------------------------------------------------- RNG
+// random_generators.f90:51-122 (L'Ecuyer 1999, five-component LFSR); the
+// Fortran ISHFT with a negative count is a logical right shift.
+static inline int64_t lsl(int64_t v, int k) { return (int64_t)((uint64_t)v << k); }
+static inline int64_t lsr(int64_t v, int k) { return (int64_t)((uint64_t)v >> k); }
+
+extern "C" void cass_rng_init(cass_rng *r, int64_t i1, int64_t i3) {
+  r->s[0] = i1;
+  r->s[1] = -759022222;
+  r->s[2] = i3;
+  r->s[3] = -1718083407;
+  r->s[4] = -123456789;
+  if ((r->s[0] & -2) == 0) r->s[0] = i1 - 8388607;
+  if ((r->s[2] & -4096) == 0) r->s[2] = i3 - 8388607;
+}
+
+extern "C" int64_t cass_rng_int(cass_rng *r) {
+  int64_t *s = r->s;
+  int64_t b;
+  b = lsr(lsl(s[0], 1) ^ s[0], 53);
+  s[0] = lsl(s[0] & -2, 10) ^ b;
+  b = lsr(lsl(s[1], 24) ^ s[1], 50);
+  s[1] = lsl(s[1] & -512, 5) ^ b;
+  b = lsr(lsl(s[2], 3) ^ s[2], 23);
+  s[2] = lsl(s[2] & -4096, 29) ^ b;
+  b = lsr(lsl(s[3], 5) ^ s[3], 24);
+  s[3] = lsl(s[3] & -131072, 23) ^ b;
+  b = lsr(lsl(s[4], 3) ^ s[4], 33);
+  s[4] = lsl(s[4] & -8388608, 8) ^ b;
+  return s[0] ^ s[1] ^ s[2] ^ s[3] ^ s[4];
+}
+
+extern "C" double cass_rng_uniform(cass_rng *r) {
+  return (double)cass_rng_int(r) * 5.4210108624275221E-20 + 0.5;
+}
+
+// accept_or_reject.f90:22-65
+extern "C" int32_t cass_accept_or_reject(cass_rng *r, double ln_pacc) {
+  if (ln_pacc <= 0.0) return 1;
+  if (ln_pacc < kMaxKBT) {
+    const double pacc = std::exp(-ln_pacc);
+    if (cass_rng_uniform(r) <= pacc) return 1;
+  }
+  return 0;
+}
+
+// --------------------------------------------------- intramolecular terms
+// Compute_Molecule_Nonbond_Intra_Energy (energy_routines.f90:980-1085), FP64
+// on the host: n_m^2/2 pairs, minimum-image free as in the reference.
+int cass_intra_host(const cass_engine *e, const cass_params &p, int s, int cbmc, const double *xyz, double *ev,
+                    double *eq, double *eqi) {
+  const int o = e->sp_atom_off[s], n = e->sp_natoms[s], po = e->sp_pair_off[s];
+  const PairConst<double> c = make_pair_const<double>(p, e->ntypes);
+  *ev = *eq = *eqi = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = ia + 1; ja < n; ++ja) {
+      const double rx = xyz[3 * ia] - xyz[3 * ja], ry = xyz[3 * ia + 1] - xyz[3 * ja + 1],
+                   rz = xyz[3 * ia + 2] - xyz[3 * ja + 2];
+      const double r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 <= p.rcut_lowsq && !e->bonded[po + ia * n + ja]) return 1;
+      bool gv, gq;
+      pair_cutoff(c, cbmc != 0, r2, gv, gq);
+      if (cbmc && e->sp_coul_cbmc[s]) gq = false;  // as written at :1062
+      const double vs = e->vdw_scale[po + ia * n + ja], qs = e->q_scale[po + ia * n + ja];
+      const int ti = e->atom_type[o + ia], tj = e->atom_type[o + ja];
+      if (gv && ti >= 0 && tj >= 0) {
+        const int k = ti * e->ntypes + tj;
+        const double eps = e->eps[k] * vs;
+        *ev = *ev + vdw_inter<double>(c, r2, eps, e->sig[k], e->mie_n[k], e->mie_m[k]);
+      }
+      if (gq) {
+        const double qi = e->atom_charge[o + ia], qj = e->atom_charge[o + ja];
+        if (p.charge_style == CASS_Q_EWALD) {
+          // Compute_AtomPair_Ewald_Real intra branch (:2115-2135)
+          const double rij = std::sqrt(r2);
+          const double erf_val = 1.0 - erfc_as<double>(p.alpha_ewald * rij);
+          const double eij = qi * qj / rij * kChargeFactor;
+          *eq = *eq + qs * eij;
+          *eqi = *eqi + (0.0 - erf_val * eij);
+        } else if (p.charge_style == CASS_Q_DSF) {
+          const double rij = std::sqrt(r2);
+          const double cfqq = qi * qj * kChargeFactor;
+          double eij = p.dsf_factor2 * (rij - p.rcut_coul) - p.dsf_factor1 + std::erfc(p.alpha_dsf * rij) / rij;
+          eij = eij * cfqq;
+          *eq = *eq + (eij - (1.0 - qs) * (cfqq / rij));
+        } else if (p.charge_style == CASS_Q_CUT || p.charge_style == CASS_Q_MINIMUM) {
+          *eq = *eq + qs * (kChargeFactor * (qi * qj) / std::sqrt(r2));
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+extern "C" int cass_mol_intra_energy(cass_engine *e, int32_t box, int32_t species, int32_t cbmc_flag,
+                                     const double *xyz, double *e_intra_vdw, double *e_intra_qq,
+                                     double *e_inter_qq, int32_t *overlap) {
+  if (!e || box < 0 || box >= (int)e->boxes.size() || species < 0 || species >= e->nspecies || !xyz)
+    return CASS_ERR_ARG;
+  *overlap = cass_intra_host(e, e->boxes[box].p, species, cbmc_flag, xyz, e_intra_vdw, e_intra_qq, e_inter_qq);
+  return CASS_OK;
+}
+
+// Compute_LR_Correction (energy_routines.f90:2956-3023)
+extern "C" int cass_lrc(cass_engine *e, int32_t box, const double *nb, double *e_lrc) {
+  if (!e || box < 0 || box >= (int)e->boxes.size() || !nb) return CASS_ERR_ARG;
+  const cass_params &p = e->boxes[box].p;
+  const int nt = e->ntypes;
+  double el = 0.0;
+  if (p.vdw_style == CASS_VDW_LJ) {
+    for (int i = 0; i < nt; ++i) {
+      double eij = 0.0;
+      for (int j = 0; j < nt; ++j) {
+        const double eps = e->eps[i * nt + j], sig = e->sig[i * nt + j];
+        const double s2 = sig * sig, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        eij = eij + nb[j] * 4.0 * eps * (s12 / (9.0 * p.rcut9) - (s6 / (3.0 * p.rcut3)));
+      }
+      el = el + nb[i] * eij;
+    }
+    el = 2.0 * kPi * el / p.volume;
+  } else if (p.vdw_style == CASS_VDW_MIE) {
+    for (int i = 0; i < nt; ++i) {
+      double eij = 0.0;
+      for (int j = 0; j < nt; ++j) {
+        const double eps = e->eps[i * nt + j], sig = e->sig[i * nt + j];
+        const double n = e->mie_n[i * nt + j], m = e->mie_m[i * nt + j];
+        const double coeff = n / (n - m) * std::pow(n / m, m / (n - m));
+        const double sr = sig / p.rcut_vdw;
+        eij = eij + nb[j] * coeff * eps * std::pow(p.rcut_vdw, 3.0) *
+                        ((std::pow(sr, n) / (3.0 - n)) + (std::pow(sr, m) / (m - 3.0)));
+      }
+      el = el + nb[i] * eij;
+    }
+    el = -2.0 * kPi * el / p.volume;
+  }
+  *e_lrc = el;
+  return CASS_OK;
+}
+
+// Compute_Molecule_Self_Energy (energy_routines.f90:2598-2645)
+extern "C" int cass_mol_self_energy(cass_engine *e, int32_t box, int32_t s, double *e_self) {
+  if (!e || box < 0 || box >= (int)e->boxes.size() || s < 0 || s >= e->nspecies) return CASS_ERR_ARG;
+  const cass_params &p = e->boxes[box].p;
+  double qq = 0.0;
+  for (int a = 0; a < e->sp_natoms[s]; ++a) {
+    const double q = e->atom_charge[e->sp_atom_off[s] + a];
+    qq = qq + q * q;
+  }
+  *e_self = 0.0;
+  if (p.charge_style == CASS_Q_EWALD) *e_self = -qq * kChargeFactor * p.alpha_ewald / kRootPi;
+  else if (p.charge_style == CASS_Q_DSF) *e_self = -qq * (p.alpha_dsf / kRootPi + p.dsf_factor1 / 2.0) * kChargeFactor;
+  return CASS_OK;
+}
+
+// ------------------------------------------------------- Widom constants
+// The insertion-independent part of Widom_Insert's energy change
+// (widom_insert.f90:212-240, 391-427): tail-correction change with the test
+// molecule's beads added, its self energy, minus the box's reciprocal
+// energy, plus the intramolecular terms of the rigid test molecule.
+int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs &w) {
+  std::memset(&w, 0, sizeof(w));
+  const int n = e->sp_natoms[species], o = e->sp_atom_off[species];
+  w.species = species;
+  w.natoms = n;
+  w.spa0 = o;
+  w.beta = b.p.beta;
+  bool charged = false;
+  for (int a = 0; a < n; ++a) {
+    w.q[a] = e->atom_charge[o + a];
+    charged = charged || w.q[a] != 0.0;
+  }
+  w.charged = charged && b.p.charge_style == CASS_Q_EWALD && b.ewald_ready;
+  if (charged && b.p.charge_style == CASS_Q_EWALD && !b.ewald_ready)
+    return cass_fail(e, CASS_ERR_STATE, "Ewald sums of the frame are not set up");
+  if ((int)e->widom_ref.size() > species && !e->widom_ref[species].empty())
+    for (int k = 0; k < 3 * n; ++k) w.ref[k] = e->widom_ref[species][k];
+  const double intra = (int)e->widom_intra.size() > species ? e->widom_intra[species] : 0.0;
+  double lrc_diff = 0.0;
+  if (b.p.vdw_sum_style == CASS_SUM_CUT_TAIL) {
+    std::vector<double> nb(e->ntypes, 0.0);
+    for (int m = 0; m < b.nmols; ++m) {
+      if (!b.live[m]) continue;
+      const int s2 = b.mol_species[m];
+      for (int a = 0; a < e->sp_natoms[s2]; ++a) {
+        const int t = e->atom_type[e->sp_atom_off[s2] + a];
+        if (t >= 0) nb[t] += 1.0;
+      }
+    }
+    double e0, e1;
+    const int32_t box = (int32_t)(&b - e->boxes.data());
+    cass_lrc(e, box, nb.data(), &e0);
+    for (int a = 0; a < n; ++a) {
+      const int t = e->atom_type[o + a];
+      if (t >= 0) nb[t] += 1.0;
+    }
+    cass_lrc(e, box, nb.data(), &e1);
+    lrc_diff = e1 - e0;
+  }
+  double e_self = 0.0;
+  if (charged) cass_mol_self_energy(e, (int32_t)(&b - e->boxes.data()), species, &e_self);
+  w.e_const = lrc_diff + e_self + intra;  // reciprocal enters as a difference (dE_recip)
+  return CASS_OK;
+}
+
+// The rigid template of a test species and its intramolecular constants are
+// set by cass_widom_set_template before batched Widom calls.
+extern "C" int cass_widom_set_template(cass_engine *e, int32_t box, int32_t species, const double *xyz,
+                                       double *e_intra_const) {
+  if (!e || box < 0 || box >= (int)e->boxes.size() || species < 0 || species >= e->nspecies || !xyz)
+    return CASS_ERR_ARG;
+  const int n = e->sp_natoms[species];
+  if (n > kMaxWidomSites) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+  cass_box_state &b = e->boxes[box];
+  double ev, eq, eqi;
+  if (cass_intra_host(e, b.p, species, 0, xyz, &ev, &eq, &eqi))
+    return cass_fail(e, CASS_ERR_ARG, "template has an intramolecular overlap");
+  *e_intra_const = ev + eq + eqi;
+  if ((int)e->widom_ref.size() < e->nspecies) {
+    e->widom_ref.resize(e->nspecies);
+    e->widom_intra.resize(e->nspecies, 0.0);
+  }
+  const double4 c = [&] {
+    TopoDev tp{};
+    tp.atom_mass = e->atom_mass.data();
+    std::vector<double> x(n), y(n), z(n);
+    for (int a = 0; a < n; ++a) {
+      x[a] = xyz[3 * a];
+      y[a] = xyz[3 * a + 1];
+      z[a] = xyz[3 * a + 2];
+    }
+    return molecule_com(tp, e->sp_atom_off[species], n, x.data(), y.data(), z.data());
+  }();
+  e->widom_ref[species].resize(3 * n);
+  for (int a = 0; a < n; ++a) {
+    e->widom_ref[species][3 * a] = xyz[3 * a] - c.x;
+    e->widom_ref[species][3 * a + 1] = xyz[3 * a + 1] - c.y;
+    e->widom_ref[species][3 * a + 2] = xyz[3 * a + 2] - c.z;
+  }
+  e->widom_intra[species] = *e_intra_const;
+  return CASS_OK;
+}

@@ -1,0 +1,218 @@
+// kernels.h -- device-side views, helpers and kernel declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pair.h"
+
+namespace cass {
+
+constexpr int kMaxBlock = 256;        // block size of the reduction kernels
+constexpr int kWidomBlock = 256;      // insertions per Widom block
+constexpr int kMaxSites = 64;         // sites per molecule (trial kernels)
+constexpr int kMaxWidomSites = 8;     // sites per Widom test molecule
+
+// Box view (per-box constants are passed by value in the kernel arguments).
+struct BoxDev {
+  const double *x, *y, *z;
+  const int *first, *species, *live;
+  const double4 *comd;
+  int nmols;
+  int cubic, vdw_minimum;
+  double L[3], hL[3];
+  double Li[9], Lc[9];   // length_inv and length, column-major
+  double rcut_max, rcut_cbmc;
+};
+
+// Topology view (nonbond tables and per species-atom data).
+struct TopoDev {
+  const double *eps, *sig, *mn, *mm;
+  const int *atom_type;
+  const double *atom_charge, *atom_mass;
+  const int *sp_natoms, *sp_atom_off, *sp_coul_cbmc;
+  int ntypes;
+};
+
+// Ewald view.
+struct EwaldDev {
+  int nvecs, nrows;
+  const double *hx, *hy, *hz, *cn;
+  double *cs, *sn, *cmol, *smol;
+  double *cs_new, *sn_new, *cm_new, *sm_new;
+  const int4 *rows;      // (ny, nz, nx0, count), k index = running sum of counts
+  const int *row_k0;     // first k of each row
+};
+
+struct WidomArgs {
+  int64_t n_ins;
+  uint64_t seed;
+  int species, natoms, spa0, charged;
+  double beta, e_const;
+  double ref[kMaxWidomSites * 3];   // site offsets from the COM (reference geometry)
+  double q[kMaxWidomSites];
+  const double *place;              // explicit coordinates [n_ins][natoms][3] or null
+};
+
+__host__ __device__ inline double rint_away(double x) { return round(x); }
+
+// Get_COM and Compute_Max_Com_Distance (get_com.f90:45-121, 220-295);
+// returns (xcom, ycom, zcom, max_dcom).
+__host__ __device__ inline double4 molecule_com(const TopoDev &tp, int off, int n, const double *x,
+                                                const double *y, const double *z) {
+  double tm = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int a = 0; a < n; ++a) {
+    const double m = tp.atom_mass[off + a];
+    tm = tm + m;
+    cx = cx + m * x[a];
+    cy = cy + m * y[a];
+    cz = cz + m * z[a];
+  }
+  cx = cx / tm;
+  cy = cy / tm;
+  cz = cz / tm;
+  double dmax = 0.0;
+  for (int a = 0; a < n; ++a) {
+    const double dx = x[a] - cx, dy = y[a] - cy, dz = z[a] - cz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    dmax = r2 > dmax ? r2 : dmax;
+  }
+  return make_double4(cx, cy, cz, sqrt(dmax));
+}
+
+// Counter-based uniform draws for batched placements (splitmix64 finaliser
+// over (seed, insertion, draw)); u in [0, 1) with 53 random bits.
+__host__ __device__ inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline double counter_uniform(uint64_t seed, uint64_t i, uint32_t d) {
+  const uint64_t h = mix64(mix64(seed ^ (i * 0xD1B54A32D192ED03ull)) + d);
+  return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Uniform COM in the box and uniform orientation (unit quaternion from three
+// uniforms, Shoemake 1992) for insertion i.
+__host__ __device__ inline void place_probe(const WidomArgs &w, const BoxDev &b, uint64_t i, double *px,
+                                            double *py, double *pz) {
+  const double u0 = counter_uniform(w.seed, i, 0), u1 = counter_uniform(w.seed, i, 1);
+  const double u2 = counter_uniform(w.seed, i, 2), u3 = counter_uniform(w.seed, i, 3);
+  const double u4 = counter_uniform(w.seed, i, 4), u5 = counter_uniform(w.seed, i, 5);
+  const double fx = u0 - 0.5, fy = u1 - 0.5, fz = u2 - 0.5;
+  const double cx = b.Lc[0] * fx + b.Lc[3] * fy + b.Lc[6] * fz;
+  const double cy = b.Lc[1] * fx + b.Lc[4] * fy + b.Lc[7] * fz;
+  const double cz = b.Lc[2] * fx + b.Lc[5] * fy + b.Lc[8] * fz;
+  const double twopi = 6.283185307179586;
+  const double r1 = sqrt(1.0 - u3), r2 = sqrt(u3);
+  const double qx = r1 * sin(twopi * u4), qy = r1 * cos(twopi * u4);
+  const double qz = r2 * sin(twopi * u5), qw = r2 * cos(twopi * u5);
+  const double R00 = 1 - 2 * (qy * qy + qz * qz), R01 = 2 * (qx * qy - qz * qw), R02 = 2 * (qx * qz + qy * qw);
+  const double R10 = 2 * (qx * qy + qz * qw), R11 = 1 - 2 * (qx * qx + qz * qz), R12 = 2 * (qy * qz - qx * qw);
+  const double R20 = 2 * (qx * qz - qy * qw), R21 = 2 * (qy * qz + qx * qw), R22 = 1 - 2 * (qx * qx + qy * qy);
+  for (int a = 0; a < w.natoms; ++a) {
+    const double rx = w.ref[3 * a], ry = w.ref[3 * a + 1], rz = w.ref[3 * a + 2];
+    px[a] = R00 * rx + R01 * ry + R02 * rz + cx;
+    py[a] = R10 * rx + R11 * ry + R12 * rz + cy;
+    pz[a] = R20 * rx + R21 * ry + R22 * rz + cz;
+  }
+}
+
+struct cplx {
+  double re, im;
+};
+__host__ __device__ inline cplx cmul(cplx a, cplx b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cplx cpow_int(cplx e, int n) {
+  cplx base = n < 0 ? cplx{e.re, -e.im} : e;  // |e| = 1: e^-1 = conj(e)
+  int k = n < 0 ? -n : n;
+  cplx r = {1.0, 0.0};
+  while (k) {
+    if (k & 1) r = cmul(r, base);
+    base = cmul(base, base);
+    k >>= 1;
+  }
+  return r;
+}
+
+// Reciprocal-space energy change of adding a test molecule:
+// sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2) * charge_factor, with
+// exp(i h.r) = Ea^nx Eb^ny Ec^nz built by phase recurrence along the k rows
+// (h.r = 2 pi n.s with s the fractional coordinates, so this holds for the
+// k sets of both branches of Ewald_Reciprocal_Lattice_Vector_Setup).
+__device__ inline double widom_recip_delta(const EwaldDev &ew, int n, const double *px, const double *py,
+                                           const double *pz, const double *q, const BoxDev &b) {
+  cplx Ea[kMaxWidomSites], Eb[kMaxWidomSites], Ec[kMaxWidomSites], Pb[kMaxWidomSites], Pc[kMaxWidomSites];
+  for (int a = 0; a < n; ++a) {
+    const double sx = b.Li[0] * px[a] + b.Li[3] * py[a] + b.Li[6] * pz[a];
+    const double sy = b.Li[1] * px[a] + b.Li[4] * py[a] + b.Li[7] * pz[a];
+    const double sz = b.Li[2] * px[a] + b.Li[5] * py[a] + b.Li[8] * pz[a];
+    double s, c;
+    sincos(kTwoPi * sx, &s, &c);
+    Ea[a] = {c, s};
+    sincos(kTwoPi * sy, &s, &c);
+    Eb[a] = {c, s};
+    sincos(kTwoPi * sz, &s, &c);
+    Ec[a] = {c, s};
+  }
+  int cur_ny = -1000000, cur_nz = -1000000;
+  double de = 0.0;
+  for (int r = 0; r < ew.nrows; ++r) {
+    const int4 row = ew.rows[r];
+    const int ny = row.x, nz = row.y, nx0 = row.z, cnt = row.w;
+    if (nz != cur_nz) {
+      for (int a = 0; a < n; ++a) Pc[a] = cpow_int(Ec[a], nz);
+      cur_nz = nz;
+      cur_ny = -1000000;
+    }
+    if (ny == cur_ny + 1) {
+      for (int a = 0; a < n; ++a) Pb[a] = cmul(Pb[a], Eb[a]);
+    } else {
+      for (int a = 0; a < n; ++a) Pb[a] = cpow_int(Eb[a], ny);
+    }
+    cur_ny = ny;
+    cplx T[kMaxWidomSites];
+    for (int a = 0; a < n; ++a) {
+      cplx base = cmul(Pb[a], Pc[a]);
+      T[a] = nx0 == 0 ? base : cmul(base, cpow_int(Ea[a], nx0));
+    }
+    const int k0 = ew.row_k0[r];
+    for (int kk = 0; kk < cnt; ++kk) {
+      double dc = 0.0, ds = 0.0;
+      for (int a = 0; a < n; ++a) {
+        dc += q[a] * T[a].re;
+        ds += q[a] * T[a].im;
+        T[a] = cmul(T[a], Ea[a]);
+      }
+      const int k = k0 + kk;
+      const double c0 = ew.cs[k], s0 = ew.sn[k];
+      de += ew.cn[k] * (2.0 * c0 * dc + dc * dc + 2.0 * s0 * ds + ds * ds);
+    }
+  }
+  return de * kChargeFactor;
+}
+
+// kernels (kernels.hip)
+template <typename T>
+__global__ void k_trials(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<T> ct, int species, int ni,
+                         const double *txyz, int exclude, int cbmc, double *part, int *part_ov);
+__global__ void k_finalize_trials(const double *part, const int *part_ov, int nblk, int ntrial, double *out,
+                                  int *out_ov);
+__global__ void k_sum(const double *in, size_t n, int stride, int comp, double *out);
+template <typename T>
+__global__ void k_total_pairs(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<T> ct, double *part,
+                              int *part_ov);
+__global__ void k_ewald_mol(BoxDev b, TopoDev tp, EwaldDev ew);
+__global__ void k_ewald_box(BoxDev b, EwaldDev ew, double *part);
+__global__ void k_ewald_move(TopoDev tp, EwaldDev ew, int m, int species, int flag, int n, const double *nxyz,
+                             double *part);
+__global__ void k_ewald_commit(EwaldDev ew, int m);
+__global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const double *txyz, double *part);
+template <typename T>
+__global__ void k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew, PairConst<double> cd,
+                        PairConst<T> ct, double *part, int64_t *part_ov, double *wvar, double *eout);
+__global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, double *out, int64_t *out_ov);
+
+}  // namespace cass

@@ -1,0 +1,490 @@
+// kernels.hip -- device code of libcassgpu for gfx950 (CDNA4, wave64).
+//
+// The energy path is a gather- and ALU-bound sum, not a contraction, so no
+// MFMA is used: every kernel is VALU work with FP64 accumulation, wave64
+// shuffle reductions and deterministic (fixed-order) cross-block sums.
+#include "kernels.h"
+
+namespace cass {
+
+// ------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__device__ inline int wave_or(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_down(v, o, 64);
+  return v;
+}
+
+// Block sum of NV doubles and an OR flag; result valid in thread 0.
+template <int NV>
+__device__ inline void block_reduce(double (&v)[NV], int &flag) {
+  __shared__ double sh[NV][kMaxBlock / 64];
+  __shared__ int shf[kMaxBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  flag = wave_or(flag);
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k][wid] = v[k];
+    shf[wid] = flag;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double s = 0.0;
+      for (int w = 0; w < nw; ++w) s += sh[k][w];
+      v[k] = s;
+    }
+    int f = 0;
+    for (int w = 0; w < nw; ++w) f |= shf[w];
+    flag = f;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- geometry
+__device__ inline void min_image(const BoxDev &b, double &dx, double &dy, double &dz) {
+  // minimum_image_separation.f90:35-112: cubic boxes shift by one box length
+  // at most; other boxes round fractional coordinates.
+  if (b.cubic) {
+    if (dx > b.hL[0]) dx -= b.L[0]; else if (dx < -b.hL[0]) dx += b.L[0];
+    if (dy > b.hL[1]) dy -= b.L[1]; else if (dy < -b.hL[1]) dy += b.L[1];
+    if (dz > b.hL[2]) dz -= b.L[2]; else if (dz < -b.hL[2]) dz += b.L[2];
+  } else {
+    double sx = b.Li[0] * dx + b.Li[3] * dy + b.Li[6] * dz;
+    double sy = b.Li[1] * dx + b.Li[4] * dy + b.Li[7] * dz;
+    double sz = b.Li[2] * dx + b.Li[5] * dy + b.Li[8] * dz;
+    sx -= rint_away(sx);
+    sy -= rint_away(sy);
+    sz -= rint_away(sz);
+    dx = b.Lc[0] * sx + b.Lc[3] * sy + b.Lc[6] * sz;
+    dy = b.Lc[1] * sx + b.Lc[4] * sy + b.Lc[7] * sz;
+    dz = b.Lc[2] * sx + b.Lc[5] * sy + b.Lc[8] * sz;
+  }
+}
+
+// Check_MoleculePair_Cutoff (energy_routines.f90:3027-3088).
+__device__ inline bool molpair_in_range(const BoxDev &b, bool cbmc, double4 ci, double4 cj) {
+  if (b.vdw_minimum) return true;
+  double dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+  min_image(b, dx, dy, dz);
+  double rcom = sqrt(dx * dx + dy * dy + dz * dz);
+  double rint = (cbmc ? b.rcut_cbmc : b.rcut_max) + ci.w + cj.w;
+  return !(rcom > rint);
+}
+
+// One probe molecule (sites px/py/pz with species-atom ids pspa) against
+// stored molecule j: Compute_MoleculePair_Energy (energy_routines.f90:1589-1730).
+// Accumulates vdW and q-q in FP64; returns true on overlap r^2 < rcut_low^2.
+// With both_cutoffs the same pass also accumulates the CBMC-cutoff sum.
+template <typename T>
+__device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const PairConst<double> &cd,
+                                      const PairConst<T> &ct, bool cbmc, bool coul_cbmc, int ni,
+                                      const double *px, const double *py, const double *pz,
+                                      const int *pspa, int j, double &ev, double &eq, double *ecbmc) {
+  const int sj = b.species[j], fj = b.first[j];
+  const int nj = tp.sp_natoms[sj], oj = tp.sp_atom_off[sj];
+  for (int ia = 0; ia < ni; ++ia) {
+    const int ti = tp.atom_type[pspa[ia]];
+    const double qi = tp.atom_charge[pspa[ia]];
+    for (int ja = 0; ja < nj; ++ja) {
+      double dx = px[ia] - b.x[fj + ja], dy = py[ia] - b.y[fj + ja], dz = pz[ia] - b.z[fj + ja];
+      min_image(b, dx, dy, dz);
+      const double r2d = dx * dx + dy * dy + dz * dz;
+      if (r2d < cd.rc_low2) return true;
+      bool gv, gq;
+      pair_cutoff(cd, cbmc, r2d, gv, gq);
+      if (cbmc && !coul_cbmc) gq = false;
+      if (!(gv || gq) && !ecbmc) continue;
+      const int tj = tp.atom_type[oj + ja];
+      const T r2 = (T)r2d;
+      T e_v = (T)0, e_q = (T)0;
+      if (ti >= 0 && tj >= 0 && (gv || (ecbmc && r2d <= cd.rc_cbmc2))) {
+        const int k = ti * tp.ntypes + tj;
+        e_v = vdw_inter<T>(ct, r2, (T)tp.eps[k], (T)tp.sig[k], (T)tp.mn[k], (T)tp.mm[k]);
+      }
+      const double qj = tp.atom_charge[oj + ja];
+      const bool any_q = ct.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
+      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<T>(ct, r2, (T)(qi * qj));
+      if (gv) ev += (double)e_v;
+      if (gq) eq += (double)e_q;
+      if (ecbmc && r2d <= cd.rc_cbmc2) {
+        bool cv = ti >= 0 && tj >= 0 && cd.vdw_style != CASS_VDW_NONE;
+        *ecbmc += (cv ? (double)e_v : 0.0) + ((any_q && coul_cbmc) ? (double)e_q : 0.0);
+      }
+    }
+  }
+  return false;
+}
+
+// ------------------------------------------------------ trial energies
+// grid (nblk, ntrial): trial t against the box's molecules
+// [blockIdx.x*blockDim.x, +blockDim.x).  Partials per (trial, block):
+// part[(t*nblk + bx)*2 + {0,1}] = {E_vdw, E_qq}, part_ov[t*nblk + bx].
+template <typename T>
+__global__ void __launch_bounds__(kMaxBlock) k_trials(BoxDev b, TopoDev tp, PairConst<double> cd,
+                                                      PairConst<T> ct, int species, int ni,
+                                                      const double *__restrict__ txyz, int exclude,
+                                                      int cbmc, double *part, int *part_ov) {
+  __shared__ double px[kMaxSites], py[kMaxSites], pz[kMaxSites];
+  __shared__ int pspa[kMaxSites];
+  __shared__ double4 pcom;
+  const int t = blockIdx.y;
+  const int oi = tp.sp_atom_off[species];
+  if (threadIdx.x < ni) {
+    const size_t o = ((size_t)t * ni + threadIdx.x) * 3;
+    px[threadIdx.x] = txyz[o];
+    py[threadIdx.x] = txyz[o + 1];
+    pz[threadIdx.x] = txyz[o + 2];
+    pspa[threadIdx.x] = oi + threadIdx.x;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) pcom = molecule_com(tp, oi, ni, px, py, pz);
+  __syncthreads();
+  const bool coul_cbmc = tp.sp_coul_cbmc[species] != 0;
+  double v[2] = {0.0, 0.0};
+  int ov = 0;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < b.nmols && j != exclude && b.live[j]) {
+    if (molpair_in_range(b, cbmc, pcom, b.comd[j]))
+      ov = molpair_energy<T>(b, tp, cd, ct, cbmc, coul_cbmc, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
+  }
+  block_reduce<2>(v, ov);
+  if (threadIdx.x == 0) {
+    const size_t o = (size_t)t * gridDim.x + blockIdx.x;
+    part[2 * o] = v[0];
+    part[2 * o + 1] = v[1];
+    part_ov[o] = ov;
+  }
+}
+
+// Fixed-order sum of per-block partials: one thread per trial.
+__global__ void k_finalize_trials(const double *part, const int *part_ov, int nblk, int ntrial,
+                                  double *out, int *out_ov) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntrial) return;
+  double a = 0.0, c = 0.0;
+  int o = 0;
+  for (int k = 0; k < nblk; ++k) {
+    const size_t i = (size_t)t * nblk + k;
+    a += part[2 * i];
+    c += part[2 * i + 1];
+    o |= part_ov[i];
+  }
+  out[2 * t] = a;
+  out[2 * t + 1] = c;
+  out_ov[t] = o;
+}
+
+// Fixed-order sum of n doubles by one block.
+__global__ void k_sum(const double *in, size_t n, int stride, int comp, double *out) {
+  double v[1] = {0.0};
+  int f = 0;
+  for (size_t i = threadIdx.x; i < n; i += blockDim.x) v[0] += in[i * stride + comp];
+  block_reduce<1>(v, f);
+  if (threadIdx.x == 0) *out = v[0];
+}
+
+// ---------------------------------------------------- total pair energy
+// Compute_System_Total_Energy inter part (energy_routines.f90:2775-2917):
+// grid (nblk, nmols); block (bx, i) covers pairs (i, j) with j > i.
+template <typename T>
+__global__ void __launch_bounds__(kMaxBlock) k_total_pairs(BoxDev b, TopoDev tp, PairConst<double> cd,
+                                                           PairConst<T> ct, double *part, int *part_ov) {
+  __shared__ double px[kMaxSites], py[kMaxSites], pz[kMaxSites];
+  __shared__ int pspa[kMaxSites];
+  const int i = blockIdx.y;
+  const int si = b.species[i], fi = b.first[i];
+  const int ni = tp.sp_natoms[si], oi = tp.sp_atom_off[si];
+  if (threadIdx.x < ni) {
+    px[threadIdx.x] = b.x[fi + threadIdx.x];
+    py[threadIdx.x] = b.y[fi + threadIdx.x];
+    pz[threadIdx.x] = b.z[fi + threadIdx.x];
+    pspa[threadIdx.x] = oi + threadIdx.x;
+  }
+  __syncthreads();
+  double v[2] = {0.0, 0.0};
+  int ov = 0;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b.live[i] && j > i && j < b.nmols && b.live[j]) {
+    if (molpair_in_range(b, false, b.comd[i], b.comd[j]))
+      ov = molpair_energy<T>(b, tp, cd, ct, false, true, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
+  }
+  block_reduce<2>(v, ov);
+  if (threadIdx.x == 0) {
+    const size_t o = (size_t)i * gridDim.x + blockIdx.x;
+    part[2 * o] = v[0];
+    part[2 * o + 1] = v[1];
+    part_ov[o] = ov;
+  }
+}
+
+// ----------------------------------------------------------------- Ewald
+// Compute_System_Ewald_Reciprocal_Energy (energy_routines.f90:4111-4231),
+// stage 1: per-molecule sums, grid (ceil(K/blk), nmols).
+__global__ void k_ewald_mol(BoxDev b, TopoDev tp, EwaldDev ew) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (k >= ew.nvecs) return;
+  double c = 0.0, s = 0.0;
+  if (b.live[m]) {
+    const int sp = b.species[m], f = b.first[m];
+    const int n = tp.sp_natoms[sp], o = tp.sp_atom_off[sp];
+    const double hx = ew.hx[k], hy = ew.hy[k], hz = ew.hz[k];
+    for (int a = 0; a < n; ++a) {
+      const double hr = hx * b.x[f + a] + hy * b.y[f + a] + hz * b.z[f + a];
+      const double q = tp.atom_charge[o + a];
+      double sv, cv;
+      sincos(hr, &sv, &cv);
+      c = c + q * cv;
+      s = s + q * sv;
+    }
+  }
+  ew.cmol[(size_t)m * ew.nvecs + k] = c;
+  ew.smol[(size_t)m * ew.nvecs + k] = s;
+}
+
+// stage 2: box sums over molecules (molecule order) and energy terms per block.
+__global__ void k_ewald_box(BoxDev b, EwaldDev ew, double *part) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[1] = {0.0};
+  int f = 0;
+  if (k < ew.nvecs) {
+    double c = 0.0, s = 0.0;
+    for (int m = 0; m < b.nmols; ++m) {
+      if (!b.live[m]) continue;
+      c = c + ew.cmol[(size_t)m * ew.nvecs + k];
+      s = s + ew.smol[(size_t)m * ew.nvecs + k];
+    }
+    ew.cs[k] = c;
+    ew.sn[k] = s;
+    v[0] = ew.cn[k] * (c * c + s * s);
+  }
+  block_reduce<1>(v, f);
+  if (threadIdx.x == 0) part[blockIdx.x] = v[0];
+}
+
+// Update_System_Ewald_Reciprocal_Energy (energy_routines.f90:2326-2497):
+// tentative sums for molecule m at new coordinates (flag: 0 translation,
+// 1 rotation, 2 intra, 3 insertion, 4 deletion).  part[blk*2+{0,1}] =
+// {sum Cn (C'^2+S'^2), sum Cn (2 C dC + dC^2 + 2 S dS + dS^2)}.
+__global__ void k_ewald_move(TopoDev tp, EwaldDev ew, int m, int species, int flag, int n,
+                             const double *__restrict__ nxyz, double *part) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  int f = 0;
+  if (k < ew.nvecs) {
+    const size_t mk = (size_t)m * ew.nvecs + k;
+    const double cm = (flag == CASS_MOVE_INSERTION) ? 0.0 : ew.cmol[mk];
+    const double sm = (flag == CASS_MOVE_INSERTION) ? 0.0 : ew.smol[mk];
+    double c = 0.0, s = 0.0;
+    if (flag != CASS_MOVE_DELETION) {
+      const int o = tp.sp_atom_off[species];
+      const double hx = ew.hx[k], hy = ew.hy[k], hz = ew.hz[k];
+      for (int a = 0; a < n; ++a) {
+        const double hr = hx * nxyz[3 * a] + hy * nxyz[3 * a + 1] + hz * nxyz[3 * a + 2];
+        const double q = tp.atom_charge[o + a];
+        double sv, cv;
+        sincos(hr, &sv, &cv);
+        c = c + q * cv;
+        s = s + q * sv;
+      }
+    }
+    const double c0 = ew.cs[k], s0 = ew.sn[k];
+    double c1, s1;
+    if (flag == CASS_MOVE_DELETION) {
+      c1 = c0 - cm;
+      s1 = s0 - sm;
+    } else if (flag == CASS_MOVE_INSERTION) {
+      c1 = c0 + c;
+      s1 = s0 + s;
+    } else {
+      c1 = c0 + (c - cm);
+      s1 = s0 + (s - sm);
+    }
+    ew.cs_new[k] = c1;
+    ew.sn_new[k] = s1;
+    ew.cm_new[k] = c;
+    ew.sm_new[k] = s;
+    const double dc = c - cm, ds = s - sm;
+    v[0] = ew.cn[k] * (c1 * c1 + s1 * s1);
+    v[1] = ew.cn[k] * (2.0 * c0 * dc + dc * dc + 2.0 * s0 * ds + ds * ds);
+  }
+  block_reduce<2>(v, f);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = v[0];
+    part[2 * blockIdx.x + 1] = v[1];
+  }
+}
+
+__global__ void k_ewald_commit(EwaldDev ew, int m) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ew.nvecs) return;
+  ew.cs[k] = ew.cs_new[k];
+  ew.sn[k] = ew.sn_new[k];
+  ew.cmol[(size_t)m * ew.nvecs + k] = ew.cm_new[k];
+  ew.smol[(size_t)m * ew.nvecs + k] = ew.sm_new[k];
+}
+
+// Update_System_Ewald_Reciprocal_Energy_Widom (energy_routines.f90:2500-2550)
+// for explicit test molecules: one block per trial, threads over k.
+__global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const double *__restrict__ txyz,
+                              double *part) {
+  __shared__ double px[kMaxSites], py[kMaxSites], pz[kMaxSites], pq[kMaxSites];
+  const int t = blockIdx.y;
+  const int o = tp.sp_atom_off[species];
+  if (threadIdx.x < n) {
+    const size_t b = ((size_t)t * n + threadIdx.x) * 3;
+    px[threadIdx.x] = txyz[b];
+    py[threadIdx.x] = txyz[b + 1];
+    pz[threadIdx.x] = txyz[b + 2];
+    pq[threadIdx.x] = tp.atom_charge[o + threadIdx.x];
+  }
+  __syncthreads();
+  double v[1] = {0.0};
+  int f = 0;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < ew.nvecs; k += gridDim.x * blockDim.x) {
+    double dc = 0.0, ds = 0.0;
+    for (int a = 0; a < n; ++a) {
+      const double hr = ew.hx[k] * px[a] + ew.hy[k] * py[a] + ew.hz[k] * pz[a];
+      double sv, cv;
+      sincos(hr, &sv, &cv);
+      dc = dc + pq[a] * cv;
+      ds = ds + pq[a] * sv;
+    }
+    const double c = ew.cs[k] + dc, s = ew.sn[k] + ds;
+    v[0] += ew.cn[k] * (c * c + s * s);
+  }
+  block_reduce<1>(v, f);
+  if (threadIdx.x == 0) part[(size_t)t * gridDim.x + blockIdx.x] = v[0];
+}
+
+// --------------------------------------------------------- Widom batch
+// The insertion loop of Widom_Insert (widom_insert.f90:319-489) for a rigid
+// single-fragment test molecule, one thread per insertion.  Placement from a
+// counter-based stream or from explicit coordinates; inter energy at the full
+// and the CBMC cutoff (overlap semantics of Build_Molecule,
+// fragment_growth.f90:519-544); reciprocal delta over the k rows by phase
+// recurrence in FP64; widom_var = exp(-beta (E_inter + dE_recip + e_const)).
+template <typename T>
+__global__ void __launch_bounds__(kWidomBlock) k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew,
+                                                       PairConst<double> cd, PairConst<T> ct,
+                                                       double *part, int64_t *part_ov, double *wvar,
+                                                       double *eout) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = w.natoms;
+  double px[kMaxWidomSites], py[kMaxWidomSites], pz[kMaxWidomSites];
+  int pspa[kMaxWidomSites];
+  double v[1] = {0.0};
+  int64_t ovl = 0;
+  if (i < w.n_ins) {
+    if (w.place) {
+      for (int a = 0; a < ni; ++a) {
+        px[a] = w.place[(i * ni + a) * 3];
+        py[a] = w.place[(i * ni + a) * 3 + 1];
+        pz[a] = w.place[(i * ni + a) * 3 + 2];
+      }
+    } else {
+      place_probe(w, b, (uint64_t)i, px, py, pz);
+    }
+    for (int a = 0; a < ni; ++a) pspa[a] = w.spa0 + a;
+    const double4 pc = molecule_com(tp, w.spa0, ni, px, py, pz);
+    double ev = 0.0, eq = 0.0, ecb = 0.0;
+    bool over = false;
+    const bool coul_cbmc = tp.sp_coul_cbmc[w.species] != 0;
+    for (int j = 0; j < b.nmols && !over; ++j) {
+      if (!b.live[j]) continue;
+      const double4 cj = b.comd[j];
+      // the full-cutoff prefilter is the wider one; the CBMC sum is taken in
+      // the same pass for the pairs within rcut_cbmc.
+      if (!molpair_in_range(b, false, pc, cj) && !molpair_in_range(b, true, pc, cj)) continue;
+      over = molpair_energy<T>(b, tp, cd, ct, false, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, &ecb);
+    }
+    double e_inter = ev + eq;
+    double de_recip = 0.0;
+    if (!over && w.charged) de_recip = widom_recip_delta(ew, ni, px, py, pz, w.q, b);
+    const double dE = e_inter + de_recip + w.e_const;
+    double wv = 0.0;
+    // Build_Molecule marks a trial whose CBMC-cutoff energy reaches
+    // max_kBT as overlapped (fragment_growth.f90:536-539)
+    const bool cbmc_over = over || (w.beta * ecb >= kMaxKBT);
+    if (!cbmc_over) wv = exp(-w.beta * dE);
+    if (cbmc_over) ovl = 1;
+    v[0] = wv;
+    if (wvar) wvar[i] = wv;
+    if (eout) {
+      eout[4 * i] = e_inter;
+      eout[4 * i + 1] = ecb;
+      eout[4 * i + 2] = de_recip;
+      eout[4 * i + 3] = over ? 1.0 : 0.0;
+    }
+  }
+  // block reduction (sum of widom_var, count of overlaps)
+  __shared__ double shv[kWidomBlock / 64];
+  __shared__ int64_t sho[kWidomBlock / 64];
+  double s = wave_sum(v[0]);
+  int64_t o = ovl;
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) o += __shfl_down(o, d, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    shv[wid] = s;
+    sho[wid] = o;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    int64_t c = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      a += shv[k];
+      c += sho[k];
+    }
+    part[blockIdx.x] = a;
+    part_ov[blockIdx.x] = c;
+  }
+}
+
+__global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, double *out,
+                            int64_t *out_ov) {
+  double v[1] = {0.0};
+  int f = 0;
+  __shared__ int64_t sh[kMaxBlock / 64];
+  int64_t c = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    v[0] += part[i];
+    c += part_ov[i];
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) c += __shfl_down(c, d, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = c;
+  block_reduce<1>(v, f);
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += sh[w];
+    *out = v[0];
+    *out_ov = t;
+  }
+}
+
+// ---------------------------------------------------------- instantiation
+template __global__ void k_trials<double>(BoxDev, TopoDev, PairConst<double>, PairConst<double>, int, int,
+                                          const double *, int, int, double *, int *);
+template __global__ void k_trials<float>(BoxDev, TopoDev, PairConst<double>, PairConst<float>, int, int,
+                                         const double *, int, int, double *, int *);
+template __global__ void k_total_pairs<double>(BoxDev, TopoDev, PairConst<double>, PairConst<double>,
+                                               double *, int *);
+template __global__ void k_total_pairs<float>(BoxDev, TopoDev, PairConst<double>, PairConst<float>,
+                                              double *, int *);
+template __global__ void k_widom<double>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,
+                                         PairConst<double>, double *, int64_t *, double *, double *);
+template __global__ void k_widom<float>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,
+                                        PairConst<float>, double *, int64_t *, double *, double *);
+
+}  // namespace cass

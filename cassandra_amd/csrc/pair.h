@@ -1,0 +1,176 @@
+// pair.h -- per-site-pair energy terms of the engine, templated on the
+// arithmetic type T of the pair math (double for the FP64 path, float for the
+// mixed path).  The caller forms the minimum-image separation in FP64 and
+// hands r^2 over in T; sums are accumulated by the caller in FP64.
+//
+// Semantics follow Compute_AtomPair_Energy (energy_routines.f90:1831-2053),
+// Compute_AtomPair_Ewald_Real (:2088-2166) with the A&S 7.1.26 erfc
+// (:2143-2163), Compute_AtomPair_DSF_Energy (:2056-2082) and
+// Check_AtomPair_Cutoff (:3092-3195).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cass_gpu.h"
+
+namespace cass {
+
+constexpr double kChargeFactor = 138935.4558;  // global_variables.f90:198
+constexpr double kRootPi = 1.7724538509;       // :184 (truncated)
+constexpr double kTwoPi = 6.2831853072;        // :183 (truncated)
+constexpr double kPi = 3.1415926536;           // :182 (truncated)
+constexpr double kMaxKBT = 708.0;              // :228
+
+// Per-box constants the pair kernels need, in the arithmetic type T.
+template <typename T>
+struct PairConst {
+  int vdw_style, vdw_sum, q_style, ntypes;
+  T rc_vdw2, rc_coul2, rc_cbmc2, rc_low2;
+  T ron2, roff2, sw1, sw2;
+  T rc_vdw;            // sqrt(rcut_vdwsq) for shifted-force / Mie
+  T rc_coul;
+  T alpha, alpha_dsf, dsf1, dsf2;
+};
+
+template <typename T>
+__host__ __device__ inline PairConst<T> make_pair_const(const cass_params &p, int ntypes) {
+  PairConst<T> c;
+  c.vdw_style = p.vdw_style;
+  c.vdw_sum = p.vdw_sum_style;
+  c.q_style = p.charge_style;
+  c.ntypes = ntypes;
+  c.rc_vdw2 = (T)p.rcut_vdwsq;
+  c.rc_coul2 = (T)p.rcut_coulsq;
+  c.rc_cbmc2 = (T)p.rcut_cbmcsq;
+  c.rc_low2 = (T)p.rcut_lowsq;
+  c.ron2 = (T)p.ron_switch_sq;
+  c.roff2 = (T)p.roff_switch_sq;
+  c.sw1 = (T)p.switch_factor1;
+  c.sw2 = (T)p.switch_factor2;
+  c.rc_vdw = (T)sqrt(p.rcut_vdwsq);
+  c.rc_coul = (T)p.rcut_coul;
+  c.alpha = (T)p.alpha_ewald;
+  c.alpha_dsf = (T)p.alpha_dsf;
+  c.dsf1 = (T)p.dsf_factor1;
+  c.dsf2 = (T)p.dsf_factor2;
+  return c;
+}
+
+__host__ __device__ inline double t_sqrt(double x) { return sqrt(x); }
+__host__ __device__ inline float t_sqrt(float x) { return sqrtf(x); }
+__host__ __device__ inline double t_exp(double x) { return exp(x); }
+__host__ __device__ inline float t_exp(float x) { return __expf(x); }
+__host__ __device__ inline double t_pow(double x, double y) { return pow(x, y); }
+__host__ __device__ inline float t_pow(float x, float y) { return powf(x, y); }
+__host__ __device__ inline double t_erfc(double x) { return erfc(x); }
+__host__ __device__ inline float t_erfc(float x) { return erfcf(x); }
+
+// A&S 7.1.26 erfc, energy_routines.f90:2143-2163.
+template <typename T>
+__host__ __device__ inline T erfc_as(T x) {
+  const T A1 = (T)0.254829592, A2 = (T)-0.284496736, A3 = (T)1.421413741, A4 = (T)-1.453152027,
+          A5 = (T)1.061405429, P = (T)0.3275911;
+  T t = (T)1 / ((T)1 + P * x);
+  T tp = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5))));
+  return tp * t_exp(-(x * x));
+}
+
+// Check_AtomPair_Cutoff (:3092-3195).
+template <typename T>
+__host__ __device__ inline void pair_cutoff(const PairConst<T> &c, bool cbmc, T r2, bool &gv, bool &gq) {
+  gv = false;
+  gq = false;
+  if (c.vdw_style == CASS_VDW_LJ) {
+    if (cbmc) gv = r2 <= c.rc_cbmc2;
+    else if (c.vdw_sum == CASS_SUM_CUT_SWITCH) gv = r2 <= c.roff2;
+    else if (c.vdw_sum == CASS_SUM_MINIMUM || c.vdw_sum == CASS_SUM_CHARMM) gv = true;
+    else gv = r2 <= c.rc_vdw2;
+  } else if (c.vdw_style == CASS_VDW_MIE) {
+    if (cbmc) gv = r2 <= c.rc_cbmc2;
+    else if (c.vdw_sum == CASS_SUM_CUT || c.vdw_sum == CASS_SUM_CUT_SHIFT || c.vdw_sum == CASS_SUM_CUT_TAIL)
+      gv = r2 <= c.rc_vdw2;
+  }
+  if (c.q_style == CASS_Q_EWALD || c.q_style == CASS_Q_DSF || c.q_style == CASS_Q_CUT)
+    gq = cbmc ? (r2 <= c.rc_cbmc2) : (r2 <= c.rc_coul2);
+  else if (c.q_style == CASS_Q_MINIMUM)
+    gq = true;
+}
+
+// Intermolecular vdW pair energy (inter branch of :1908-2006).
+template <typename T>
+__host__ __device__ inline T vdw_inter(const PairConst<T> &c, T r2, T eps, T sig, T mn, T mm) {
+  if (c.vdw_style == CASS_VDW_LJ) {
+    T s2 = (sig * sig) / r2;
+    T s6 = s2 * s2 * s2;
+    T s12 = s6 * s6;
+    T e = (T)4 * eps * (s12 - s6);
+    switch (c.vdw_sum) {
+      case CASS_SUM_CUT_SHIFT: {
+        T c2 = sig * sig / c.rc_vdw2, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        e = e - (T)4 * eps * (c12 - c6);
+      } break;
+      case CASS_SUM_CUT_SWITCH:
+        if (r2 < c.ron2) {
+        } else if (r2 <= c.roff2) {
+          T d = c.roff2 - r2;
+          e = (d * d) * (c.sw2 + (T)2 * r2) * c.sw1 * e;
+        } else {
+          e = (T)0;
+        }
+        break;
+      case CASS_SUM_CHARMM:
+        e = eps * (s12 - (T)2 * s6);
+        break;
+      case CASS_SUM_CUT_SHIFT_FORCE: {
+        T c2 = sig * sig / c.rc_vdw2, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        e = e - (T)4 * eps * (c12 - c6);
+        T rij = t_sqrt(r2), rc = c.rc_vdw;
+        T de = (T)-24 * eps * ((T)2 * c12 / rc - c6 / rc);
+        e = e - (rij - rc) * de;
+      } break;
+      default:
+        break;
+    }
+    return e;
+  }
+  if (c.vdw_style == CASS_VDW_MIE) {
+    T rij = t_sqrt(r2);
+    T coeff = mn / (mn - mm) * t_pow(mn / mm, mm / (mn - mm));
+    T sr = sig / rij;
+    T e = coeff * eps * (t_pow(sr, mn) - t_pow(sr, mm));
+    if (c.vdw_sum == CASS_SUM_CUT_SHIFT) {
+      T sc = sig / c.rc_vdw;
+      e = e - coeff * eps * (t_pow(sc, mn) - t_pow(sc, mm));
+    }
+    return e;
+  }
+  return (T)0;
+}
+
+// Intermolecular q-q pair energy.  In FP64 the Ewald term keeps the
+// reference's form Eij - (1 - erfc)*Eij; in FP32 it is evaluated as the
+// mathematically equal Eij*erfc, which does not cancel.
+template <typename T>
+__host__ __device__ inline T qq_inter(const PairConst<T> &c, T r2, T qiqj) {
+  if (c.q_style == CASS_Q_EWALD) {
+    T rij = t_sqrt(r2);
+    T ec = erfc_as(c.alpha * rij);
+    T eqq = qiqj / rij * (T)kChargeFactor;
+    if (sizeof(T) == sizeof(double)) {
+      T erf_val = (T)1 - ec;
+      return eqq - erf_val * eqq;
+    }
+    return eqq * ec;
+  }
+  if (c.q_style == CASS_Q_DSF) {
+    T rij = t_sqrt(r2);
+    T e = c.dsf2 * (rij - c.rc_coul) - c.dsf1 + t_erfc(c.alpha_dsf * rij) / rij;
+    return e * (qiqj * (T)kChargeFactor);
+  }
+  if (c.q_style == CASS_Q_CUT || c.q_style == CASS_Q_MINIMUM) {
+    return (T)kChargeFactor * qiqj / t_sqrt(r2);
+  }
+  return (T)0;
+}
+
+}  // namespace cass

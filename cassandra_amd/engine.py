@@ -1,0 +1,344 @@
+"""Python host layer over libcassgpu.so (the C ABI in include/cass_gpu.h).
+
+Mirrors the reference's energy interface one box at a time: the methods are
+named after the Fortran subroutines they replace (energy_routines.f90,
+fragment_growth.f90, widom_insert.f90) and take the same arguments, with
+0-based molecule and species indices.  There is no CPU fallback: if the HIP
+library is missing or no GPU is visible, construction raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from . import systems as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcassgpu.so")
+
+FP64 = 0
+MIXED = 1
+
+_STATUS = {0: "ok", 1: "bad argument", 2: "HIP error", 3: "out of device memory", 4: "too many k vectors",
+           5: "net charge", 6: "call out of order", 7: "no GPU visible", 8: "cutoff exceeds half the box"}
+
+
+class CassError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libcassgpu.so and declares the C ABI.  Raises if it is missing."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is not built; run __graft_entry__.build() or make -C cassandra_amd/csrc")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    d, i32, i64, u64, vp = C.c_double, C.c_int32, C.c_int64, C.c_uint64, C.c_void_p
+    sig = {
+        "cass_params_derive": (i32, [P(abi.CassParams)]),
+        "cass_engine_create": (i32, [i32, P(abi.CassTopology), i32, P(vp)]),
+        "cass_engine_destroy": (i32, [vp]),
+        "cass_set_precision": (i32, [vp, i32]),
+        "cass_last_error": (C.c_char_p, [vp]),
+        "cass_last_kernel_ms": (i32, [vp, P(d)]),
+        "cass_box_create": (i32, [vp, P(abi.CassParams), i32, P(i32)]),
+        "cass_box_upload": (i32, [vp, i32, i32, P(i32), P(i32), P(d), P(d), P(d)]),
+        "cass_mol_update": (i32, [vp, i32, i32, i32, P(d), P(d), P(d)]),
+        "cass_box_download": (i32, [vp, i32, P(d)]),
+        "cass_mol_inter_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(d), P(i32)]),
+        "cass_mol_intra_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(i32)]),
+        "cass_trial_energies": (i32, [vp, i32, i32, i32, P(d), i32, i32, P(d), P(i32)]),
+        "cass_total_energy": (i32, [vp, i32, i32, P(abi.CassEnergy), P(i32)]),
+        "cass_lrc": (i32, [vp, i32, P(d), P(d)]),
+        "cass_mol_self_energy": (i32, [vp, i32, i32, P(d)]),
+        "cass_ewald_setup": (i32, [vp, i32, P(i32)]),
+        "cass_ewald_full": (i32, [vp, i32, P(d)]),
+        "cass_ewald_move": (i32, [vp, i32, i32, i32, P(d), P(d), P(d)]),
+        "cass_ewald_commit": (i32, [vp, i32]),
+        "cass_ewald_revert": (i32, [vp, i32]),
+        "cass_ewald_widom": (i32, [vp, i32, i32, i32, P(d), P(d)]),
+        "cass_widom_batch": (i32, [vp, i32, i32, i64, u64, P(d), P(abi.CassWidomResult)]),
+        "cass_widom_set_template": (i32, [vp, i32, i32, P(d), P(d)]),
+        "cass_widom_placements": (i32, [vp, i32, i32, i64, P(d), P(d), P(d), P(abi.CassWidomResult)]),
+        "cass_rng_init": (None, [P(abi.CassRng), i64, i64]),
+        "cass_rng_uniform": (d, [P(abi.CassRng)]),
+        "cass_rng_int": (i64, [P(abi.CassRng)]),
+        "cass_accept_or_reject": (i32, [P(abi.CassRng), d]),
+        "cass_nvt_run": (i32, [vp, i32, P(abi.CassRng), P(abi.CassNvtOptions), i64, P(abi.CassEnergy),
+                               P(abi.CassNvtRecord)]),
+    }
+    optional = {
+        "cass_trial_energies_multi": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i64), P(i64), P(d), i32, P(i32),
+                                            P(d), P(i32)]),
+        "cass_adjoint": (i32, [vp, i32, P(d), P(d), P(d), P(d), P(d)]),
+    }
+    for name, (res, args) in list(sig.items()) + list(optional.items()):
+        if not hasattr(lib, name):
+            if name in optional:
+                continue
+            raise ImportError(f"{path} does not export {name}")
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def derive(system: S.System) -> abi.CassParams:
+    """cass_params of a box (raw block from the System, derived block by
+    cass_params_derive)."""
+    p = abi.raw_params(system)
+    rc = load_library().cass_params_derive(C.byref(p))
+    if rc:
+        raise CassError(rc, "cass_params_derive")
+    return p
+
+
+def _p(a, t=C.c_double):
+    return None if a is None else a.ctypes.data_as(C.POINTER(t))
+
+
+class Engine:
+    """One engine (one GPU, one topology) holding one or more boxes."""
+
+    def __init__(self, system: S.System, max_boxes=1, device=0, precision=FP64):
+        self.lib = load_library()
+        self.topo = abi.Topology(system)
+        self.system = system
+        h = C.c_void_p()
+        rc = self.lib.cass_engine_create(device, C.byref(self.topo.c), max_boxes, C.byref(h))
+        if rc:
+            raise CassError(rc, "cass_engine_create (is a GPU visible?)")
+        self.h = h
+        self.boxes = []
+        self.set_precision(precision)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cass_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc:
+            raise CassError(rc, f"{what}: {self.lib.cass_last_error(self.h).decode()}")
+
+    def last_kernel_ms(self):
+        ms = C.c_double()
+        self._chk(self.lib.cass_last_kernel_ms(self.h, C.byref(ms)), "cass_last_kernel_ms")
+        return ms.value
+
+    def set_precision(self, prec):
+        self._chk(self.lib.cass_set_precision(self.h, prec), "cass_set_precision")
+        self.precision = prec
+
+    def add_box(self, system: S.System, max_mols=None, fold=True):
+        """cass_box_create + cass_box_upload of a System (coordinates folded
+        into the central box as the reference's start-up does, main.f90:367-376)."""
+        p = derive(system)
+        spec, live, xyz = abi.box_arrays(system)
+        cap = max(int(max_mols or 0), len(spec), 1)
+        bid = C.c_int32()
+        self._chk(self.lib.cass_box_create(self.h, C.byref(p), cap, C.byref(bid)), "cass_box_create")
+        box = Box(self, bid.value, system, p)
+        self.boxes.append(box)
+        if fold:
+            xyz = box.fold(spec, xyz)
+        box.upload(spec, live, xyz)
+        return box
+
+
+class Box:
+    def __init__(self, eng: Engine, bid, system, params):
+        self.eng, self.id, self.system, self.params = eng, bid, system, params
+        self.lib = eng.lib
+
+    # ---------------------------------------------------------------- state
+    def fold(self, spec, xyz):
+        """Fold_Molecule for every molecule of a cubic box (COM into the box)."""
+        if self.system.box_shape != "cubic":
+            return xyz
+        xyz = xyz.copy()
+        L = np.array(self.system.box)
+        k = 0
+        for s in spec:
+            sp = self.system.species[s]
+            n = sp.natoms
+            sites = xyz[3 * k:3 * (k + n)].reshape(n, 3)
+            com = S.com_of(sp, sites)
+            for d in range(3):
+                if com[d] > 0.5 * L[d]:
+                    sites[:, d] -= L[d]
+                elif com[d] < -0.5 * L[d]:
+                    sites[:, d] += L[d]
+            xyz[3 * k:3 * (k + n)] = sites.ravel()
+            k += n
+        return xyz
+
+    def upload(self, spec, live, xyz, com=None, max_dcom=None):
+        self.spec = np.ascontiguousarray(spec, dtype=np.int32)
+        self.live = np.ascontiguousarray(live, dtype=np.int32)
+        self.nmols = len(spec)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.nsites = xyz.size // 3
+        self.eng._chk(self.lib.cass_box_upload(self.eng.h, self.id, self.nmols, _p(self.spec, C.c_int32),
+                                               _p(self.live, C.c_int32), _p(xyz), _p(com), _p(max_dcom)),
+                      "cass_box_upload")
+
+    def download(self):
+        out = np.zeros(3 * self.nsites)
+        self.eng._chk(self.lib.cass_box_download(self.eng.h, self.id, _p(out)), "cass_box_download")
+        return out
+
+    def mol_update(self, mol, xyz, live=True, com=None, max_dcom=None):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        c = None if com is None else np.ascontiguousarray(com, dtype=np.float64)
+        d = None if max_dcom is None else np.array([max_dcom], dtype=np.float64)
+        self.eng._chk(self.lib.cass_mol_update(self.eng.h, self.id, mol, int(live), _p(xyz), _p(c), _p(d)),
+                      "cass_mol_update")
+
+    # ------------------------------------------------------------- energies
+    def compute_molecule_nonbond_inter_energy(self, mol, cbmc=False, xyz=None):
+        ev, eq, ov = C.c_double(), C.c_double(), C.c_int32()
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64)
+        self.eng._chk(self.lib.cass_mol_inter_energy(self.eng.h, self.id, mol, int(cbmc), _p(x), None, None,
+                                                     C.byref(ev), C.byref(eq), C.byref(ov)),
+                      "cass_mol_inter_energy")
+        return ev.value, eq.value, bool(ov.value)
+
+    def compute_molecule_nonbond_intra_energy(self, species, xyz, cbmc=False):
+        a, b, c, ov = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        self.eng._chk(self.lib.cass_mol_intra_energy(self.eng.h, self.id, species, int(cbmc), _p(x), C.byref(a),
+                                                     C.byref(b), C.byref(c), C.byref(ov)),
+                      "cass_mol_intra_energy")
+        return a.value, b.value, c.value, bool(ov.value)
+
+    def trial_energies(self, species, trial_xyz, cbmc=False, exclude=-1):
+        t = np.ascontiguousarray(trial_xyz, dtype=np.float64)
+        n = t.shape[0]
+        nrg = np.zeros(n)
+        ov = np.zeros(n, dtype=np.int32)
+        self.eng._chk(self.lib.cass_trial_energies(self.eng.h, self.id, species, n, _p(t), int(cbmc), exclude,
+                                                   _p(nrg), _p(ov, C.c_int32)), "cass_trial_energies")
+        return nrg, ov.astype(bool)
+
+    def compute_system_total_energy(self, intra=True):
+        e = abi.CassEnergy()
+        ov = C.c_int32()
+        self.eng._chk(self.lib.cass_total_energy(self.eng.h, self.id, int(intra), C.byref(e), C.byref(ov)),
+                      "cass_total_energy")
+        if ov.value:
+            raise CassError(6, "overlap in the configuration")
+        return e.as_dict()
+
+    def lrc(self, nbeads):
+        nb = np.ascontiguousarray(nbeads, dtype=np.float64)
+        out = C.c_double()
+        self.eng._chk(self.lib.cass_lrc(self.eng.h, self.id, _p(nb), C.byref(out)), "cass_lrc")
+        return out.value
+
+    def molecule_self_energy(self, species):
+        out = C.c_double()
+        self.eng._chk(self.lib.cass_mol_self_energy(self.eng.h, self.id, species, C.byref(out)),
+                      "cass_mol_self_energy")
+        return out.value
+
+    # ----------------------------------------------------------------- Ewald
+    def ewald_setup(self):
+        n = C.c_int32()
+        self.eng._chk(self.lib.cass_ewald_setup(self.eng.h, self.id, C.byref(n)), "cass_ewald_setup")
+        return n.value
+
+    def ewald_full(self):
+        e = C.c_double()
+        self.eng._chk(self.lib.cass_ewald_full(self.eng.h, self.id, C.byref(e)), "cass_ewald_full")
+        return e.value
+
+    def ewald_move(self, mol, flag, xyz=None):
+        e, de = C.c_double(), C.c_double()
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64)
+        self.eng._chk(self.lib.cass_ewald_move(self.eng.h, self.id, mol, flag, _p(x), C.byref(e), C.byref(de)),
+                      "cass_ewald_move")
+        return e.value, de.value
+
+    def ewald_commit(self):
+        self.eng._chk(self.lib.cass_ewald_commit(self.eng.h, self.id), "cass_ewald_commit")
+
+    def ewald_revert(self):
+        self.eng._chk(self.lib.cass_ewald_revert(self.eng.h, self.id), "cass_ewald_revert")
+
+    def ewald_widom(self, species, trial_xyz):
+        t = np.ascontiguousarray(trial_xyz, dtype=np.float64)
+        out = np.zeros(t.shape[0])
+        self.eng._chk(self.lib.cass_ewald_widom(self.eng.h, self.id, species, t.shape[0], _p(t), _p(out)),
+                      "cass_ewald_widom")
+        return out
+
+    # ----------------------------------------------------------------- Widom
+    def widom_set_template(self, species, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        e = C.c_double()
+        self.eng._chk(self.lib.cass_widom_set_template(self.eng.h, self.id, species, _p(x), C.byref(e)),
+                      "cass_widom_set_template")
+        return e.value
+
+    def widom_batch(self, species, n_ins, seed, keep=False):
+        r = abi.CassWidomResult()
+        wv = np.zeros(n_ins) if keep else None
+        self.eng._chk(self.lib.cass_widom_batch(self.eng.h, self.id, species, n_ins, seed, _p(wv), C.byref(r)),
+                      "cass_widom_batch")
+        return r, wv
+
+    def widom_placements(self, species, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = x.shape[0]
+        wv = np.zeros(n)
+        eo = np.zeros((n, 4))
+        r = abi.CassWidomResult()
+        self.eng._chk(self.lib.cass_widom_placements(self.eng.h, self.id, species, n, _p(x), _p(wv), _p(eo),
+                                                     C.byref(r)), "cass_widom_placements")
+        return r, wv, eo
+
+    # ------------------------------------------------------------------- NVT
+    def nvt_run(self, rng: abi.CassRng, nsteps, energy: dict, species=0, record=True):
+        sp = self.system.species[species]
+        opt = abi.CassNvtOptions(self.params_prob_translate(), self.system.max_disp, self.system.max_rot_radians(),
+                                 int(sp.linear), species)
+        e = abi.CassEnergy(**{k: energy.get(k, 0.0) for k in abi.ENERGY_FIELDS})
+        kind = np.zeros(nsteps, dtype=np.uint8)
+        acc = np.zeros(nsteps, dtype=np.uint8)
+        et = np.zeros(nsteps)
+        lp = np.zeros(nsteps)
+        rs = np.zeros(5 * nsteps, dtype=np.int64)
+        rec = abi.CassNvtRecord(_p(kind, C.c_uint8), _p(acc, C.c_uint8), _p(et), _p(rs, C.c_int64), _p(lp))
+        self.eng._chk(self.lib.cass_nvt_run(self.eng.h, self.id, C.byref(rng), C.byref(opt), nsteps, C.byref(e),
+                                            C.byref(rec) if record else None), "cass_nvt_run")
+        return dict(kind=kind, accepted=acc.astype(bool), energy=et, ln_pacc=lp, rng=rs.reshape(-1, 5),
+                    final=e.as_dict())
+
+    def params_prob_translate(self):
+        return self.system.prob_translate
+
+
+def rng(i1, i3) -> abi.CassRng:
+    """init_seeds (random_generators.f90:51-90) with one thread."""
+    r = abi.CassRng()
+    load_library().cass_rng_init(C.byref(r), i1, i3)
+    return r

@@ -1,0 +1,219 @@
+"""Cassandra on-disk formats, host side: MCF molecule files, the ``.inp``
+input file (the sections the energy path reads) and ``.xyz`` frames.
+
+Writers emit every real number with ``repr`` so that the reference's
+list-directed reads (input_routines.f90, String_To_Double) and this package
+see bit-identical doubles.  Readers cover the subset of sections that fix the
+energy path: Atom_Info / Bond_Info / Angle_Info / Dihedral_Info /
+Intra_Scaling of an MCF, and ``# VDW_Style``, ``# Charge_Style``,
+``# Rcutoff_Low``, ``# Box_Info``, ``# Temperature_Info``, ``# Seed_Info``,
+``# Molecule_Files``, ``# Mixing_Rule`` and ``# CBMC_Info`` of an input file
+(input_routines.f90:425-812, 1550-1600, 3784-3830, 4085-4100, 5931-6171).
+"""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from .systems import Atom, Species, System
+
+
+def _r(x):
+    """Decimal text of x with no exponent, as the writers emit every number
+    of an input or MCF file."""
+    x = float(x)
+    if x == int(x) and abs(x) < 1e15:
+        return f"{x:.1f}"
+    t = f"{x:.17f}".rstrip("0")
+    return t if not t.endswith(".") else t + "0"
+
+
+def _string_to_int(s: str) -> int:
+    s = s.strip()
+    if not s or s in ("-", "+"):
+        return 0
+    return int(s)
+
+
+def _pow10_single(e: int) -> float:
+    """Default-REAL 10.0**e as the reference evaluates it (single precision,
+    integer power by repeated multiplication, reciprocal for e < 0)."""
+    v = np.float32(1.0)
+    for _ in range(abs(e)):
+        v = np.float32(v * np.float32(10.0))
+    if e < 0:
+        v = np.float32(np.float32(1.0) / v)
+    return float(v)
+
+
+def string_to_double(text: str) -> float:
+    """Restatement of the reference's String_To_Double (io_utilities.f90:331-429):
+    the integer part, plus each fractional digit divided by 10**k summed in
+    double precision, times a single-precision power of ten for an exponent.
+    The reference reads every real of an input or MCF file with it."""
+    s = text.strip()
+    neg = s.startswith("-")
+    exp_start = dec_start = -1
+    for i, ch in enumerate(s):
+        if ch == ".":
+            dec_start = i
+        if ch in "DdEe":
+            exp_start = i
+    if exp_start < 0:
+        exp_start = len(s)
+        expv = "0"
+    else:
+        expv = s[exp_start + 1:]
+    if dec_start < 0:
+        dec_start = exp_start
+        frac = "0"
+    else:
+        frac = s[dec_start + 1:exp_start]
+    val = float(_string_to_int(s[:dec_start]))
+    div = 10.0
+    for ch in frac.rstrip(" "):
+        add = (ord(ch) - 48) / div
+        if neg:
+            add = -add
+        val = val + add
+        div = div * 10.0
+    return val * _pow10_single(_string_to_int(expv))
+
+
+def parsed(x) -> float:
+    """The value the reference holds after reading x written by these writers."""
+    return string_to_double(_r(x))
+
+
+def write_mcf(sp: Species, path: str, with_fragment=False):
+    lines = ["# Atom_Info", str(sp.natoms)]
+    for i, a in enumerate(sp.atoms):
+        if a.mie_n:
+            lines.append(f"{i+1} {a.name} {a.element} {_r(a.mass)} {_r(a.charge)} Mie "
+                         f"{_r(a.eps_K)} {_r(a.sigma)} {_r(a.mie_n)} {_r(a.mie_m)}")
+        else:
+            lines.append(f"{i+1} {a.name} {a.element} {_r(a.mass)} {_r(a.charge)} LJ "
+                         f"{_r(a.eps_K)} {_r(a.sigma)}")
+    lines += ["", "# Bond_Info", str(len(sp.bonds))]
+    for k, (i, j, length) in enumerate(sp.bonds):
+        lines.append(f"{k+1} {i+1} {j+1} fixed {_r(length)}")
+    lines += ["", "# Angle_Info", str(len(sp.angles))]
+    for k, (i, j, l, kind, params) in enumerate(sp.angles):
+        lines.append(f"{k+1} {i+1} {j+1} {l+1} {kind} " + " ".join(_r(p) for p in params))
+    lines += ["", "# Dihedral_Info", str(len(sp.dihedrals))]
+    for k, (i, j, l, m, kind, params) in enumerate(sp.dihedrals):
+        lines.append(f"{k+1} {i+1} {j+1} {l+1} {m+1} {kind} " + " ".join(_r(p) for p in params))
+    lines += ["", "# Improper_Info", "0", "", "# Intra_Scaling",
+              " ".join(_r(x) for x in sp.vdw_scale), " ".join(_r(x) for x in sp.q_scale), ""]
+    if with_fragment and sp.natoms > 0:
+        lines += ["# Fragment_Info", "1", f"1 {sp.natoms} " + " ".join(str(i + 1) for i in range(sp.natoms)),
+                  "", "# Fragment_Connectivity", "0", ""]
+    else:
+        lines += ["# Fragment_Info", "0", "", "# Fragment_Connectivity", "0", ""]
+    lines.append("END")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+def write_xyz(system: System, path: str):
+    rows = []
+    for sp, xyz in zip(system.species, system.coords):
+        for m in range(xyz.shape[0]):
+            for ia, a in enumerate(sp.atoms):
+                x, y, z = xyz[m, ia]
+                rows.append(f"{a.element} {_r(x)} {_r(y)} {_r(z)}")
+    with open(path, "w") as f:
+        f.write(f"{len(rows)}\n\n" + "\n".join(rows) + "\n")
+
+
+def read_xyz(path: str, system: System):
+    """Fill ``system.coords`` from an .xyz frame laid out species-major,
+    molecule-major, atom-minor (load_next_frame.f90 / Read_Config order)."""
+    with open(path) as f:
+        n = int(f.readline().split()[0])
+        f.readline()
+        xyz = np.array([[float(v) for v in f.readline().split()[1:4]] for _ in range(n)])
+    k = 0
+    coords = []
+    for sp, nm in zip(system.species, system.nmols):
+        c = xyz[k:k + nm * sp.natoms].reshape(nm, sp.natoms, 3)
+        coords.append(c)
+        k += nm * sp.natoms
+    system.coords = coords
+    return system
+
+
+def write_inp(system: System, path: str, run_name: str, mcf_names, xyz_name,
+              nsteps=0, with_cbmc=True, verbose=False):
+    s = system
+    vdw = {"lj": "LJ", "mie": "Mie", "none": "none"}[s.vdw_style]
+    if s.vdw_style == "none":
+        vdw_line = "none"
+    elif s.vdw_sum == "cut_switch":
+        vdw_line = f"{vdw} cut_switch {_r(s.ron_switch)} {_r(s.roff_switch)}"
+    elif s.vdw_sum == "charmm":
+        vdw_line = f"{vdw} CHARMM {_r(s.ron_switch)} {_r(s.roff_switch)}"
+    elif s.vdw_sum == "minimum_image":
+        vdw_line = f"{vdw} minimum_image"
+    else:
+        vdw_line = f"{vdw} {s.vdw_sum} {_r(s.rcut_vdw)}"
+    if s.charge_style == "none":
+        q_line = "none"
+    elif s.charge_style == "ewald":
+        q_line = f"coul ewald {_r(s.rcut_coul)} {_r(s.ewald_tol)}"
+    elif s.charge_style == "dsf":
+        q_line = f"coul dsf {_r(s.rcut_coul)}" + ("" if s.alpha_dsf is None else f" {_r(s.alpha_dsf)}")
+    elif s.charge_style == "cut":
+        q_line = f"coul cut {_r(s.rcut_coul)}"
+    else:
+        q_line = "coul minimum_image"
+    if s.box_shape == "cubic":
+        box_lines = ["1", "cubic", _r(s.box[0])]
+    else:
+        box_lines = ["1", "orthogonal", f"{_r(s.box[0])} {_r(s.box[1])} {_r(s.box[2])}"]
+    nsp = len(s.species)
+    out = ["# Run_Name", run_name, "",
+           "# Sim_Type", "nvt_mc", "",
+           "# Nbr_Species", str(nsp), "",
+           "# VDW_Style", vdw_line, "",
+           "# Charge_Style", q_line, "",
+           "# Mixing_Rule", s.mixing, "",
+           "# Seed_Info", f"{s.seed[0]} {s.seed[1]}", "",
+           "# Rcutoff_Low", _r(s.rcut_low), "",
+           "# Molecule_Files"]
+    for name, sp, nm in zip(mcf_names, s.species, s.nmols):
+        out.append(f"{name} {max(nm, 1)}")
+    out += ["", "# Box_Info", *box_lines, "",
+            "# Temperature_Info", _r(s.temperature), "",
+            "# Move_Probability_Info", "",
+            "# Prob_Translation", _r(s.prob_translate), " ".join(_r(s.max_disp) for _ in range(nsp)), ""]
+    if s.prob_translate < 1.0:
+        out += ["# Prob_Rotation", _r(1.0 - s.prob_translate),
+                " ".join(_r(s.max_rot) for _ in range(nsp)), ""]
+    out += ["# Done_Probability_Info", "",
+            "# Start_Type", "read_config " + " ".join(str(n) for n in s.nmols) + f" {xyz_name}", "",
+            "# Run_Type", "production 1000000", "",
+            "# Simulation_Length_Info", "units steps", "prop_freq 1000000", "coord_freq 1000000",
+            f"run {nsteps}", ""]
+    if with_cbmc:
+        out += ["# CBMC_Info", "kappa_ins 12", f"rcut_cbmc {_r(s.rcut_cbmc)}", ""]
+    if verbose:
+        out += ["# Verbose_Logfile", "true", ""]
+    out.append("END")
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def write_case(system: System, directory: str, run_name="case", nsteps=0, with_cbmc=True):
+    """Write .inp + .mcf + .xyz for one box into ``directory``; returns the .inp path."""
+    os.makedirs(directory, exist_ok=True)
+    mcfs = []
+    for k, sp in enumerate(system.species):
+        name = f"species{k+1}_{sp.name}.mcf"
+        write_mcf(sp, os.path.join(directory, name))
+        mcfs.append(name)
+    write_xyz(system, os.path.join(directory, "start.xyz"))
+    inp = os.path.join(directory, run_name + ".inp")
+    write_inp(system, inp, run_name + ".out", mcfs, "start.xyz", nsteps=nsteps, with_cbmc=with_cbmc)
+    return inp

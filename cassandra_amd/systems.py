@@ -1,0 +1,445 @@
+"""Box / species / force-field description shared by the host layer, the
+benchmark and the golden-vector generator.
+
+Units follow the reference (global_variables.f90:182-228): lengths in
+Angstrom, energies in amu A^2/ps^2 (10 J/mol), epsilon given in K and
+converted with ``kboltz`` exactly as input_routines.f90:1573 does, charges in
+e with ``charge_factor`` folded in at the pair level.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+# Constants of the reference (global_variables.f90:182-228), including the
+# truncated pi values that enter Cn, the self energy and the tail correction.
+PI = 3.1415926536
+TWO_PI = 6.2831853072
+ROOT_PI = 1.7724538509
+KBOLTZ = 0.8314472
+H_PLANCK = 39.9031268605
+CHARGE_FACTOR = 138935.4558
+ATOMIC_TO_KJMOL = 0.01
+TINY_NUMBER = 0.0000001
+MAX_KBT = 708.0
+MAXK = 100000
+
+# style codes shared with include/cass_gpu.h
+VDW_NONE, VDW_LJ, VDW_MIE = 0, 1, 2
+SUM_CUT, SUM_CUT_TAIL, SUM_CUT_SHIFT, SUM_CUT_SWITCH, SUM_CUT_SHIFT_FORCE, SUM_CHARMM, SUM_MINIMUM = range(7)
+Q_NONE, Q_EWALD, Q_DSF, Q_CUT, Q_MINIMUM = range(5)
+BOX_CUBIC, BOX_ORTHO, BOX_CELL = 0, 1, 2
+
+_VDW_SUM_NAMES = {"cut": SUM_CUT, "cut_tail": SUM_CUT_TAIL, "cut_shift": SUM_CUT_SHIFT,
+                  "cut_switch": SUM_CUT_SWITCH, "cut_shift-force": SUM_CUT_SHIFT_FORCE,
+                  "charmm": SUM_CHARMM, "minimum_image": SUM_MINIMUM}
+_Q_NAMES = {"none": Q_NONE, "ewald": Q_EWALD, "dsf": Q_DSF, "cut": Q_CUT, "minimum_image": Q_MINIMUM}
+
+
+@dataclass
+class Atom:
+    name: str          # atom-type name (sites of one name share one type)
+    element: str
+    mass: float
+    charge: float
+    eps_K: float = 0.0
+    sigma: float = 0.0
+    mie_n: float = 0.0
+    mie_m: float = 0.0
+
+
+@dataclass
+class Species:
+    name: str
+    atoms: List[Atom]
+    ref_coords: np.ndarray                 # (natoms, 3) rigid reference geometry
+    bonds: List[tuple] = field(default_factory=list)       # (i, j, length) 0-based
+    angles: List[tuple] = field(default_factory=list)      # (i, j, k, 'fixed'|'harmonic', params)
+    dihedrals: List[tuple] = field(default_factory=list)   # (i, j, k, l, 'OPLS', a0..a3 kJ/mol)
+    vdw_scale: tuple = (0.0, 0.0, 0.0, 1.0)                # 1-2, 1-3, 1-4, 1-N
+    q_scale: tuple = (0.0, 0.0, 0.0, 1.0)
+    fragments: Optional[list] = None       # list of atom-index lists; None = one rigid fragment
+    fragment_bonds: Optional[list] = None  # (frag_a, frag_b) connectivity
+
+    @property
+    def natoms(self):
+        return len(self.atoms)
+
+    @property
+    def molecular_weight(self):
+        return sum(a.mass for a in self.atoms)
+
+    @property
+    def total_charge(self):
+        return sum(a.charge for a in self.atoms)
+
+    @property
+    def linear(self):
+        """Linear molecules use Euler rotations in the reference (move_rotate.f90:211)."""
+        x = self.ref_coords - self.ref_coords.mean(0)
+        if self.natoms <= 2:
+            return True
+        s = np.linalg.svd(x, compute_uv=False)
+        return s[1] < 1e-8 * max(s[0], 1.0)
+
+    def intra_scale_tables(self):
+        """vdw_intra_scale / charge_intra_scale (ia, ja): 1-N everywhere, then
+        1-4 pairs from the dihedral list, 1-3 from the angle list and 1-2 from
+        the bond list, each overriding the last (create_intra_exclusion_table.f90)."""
+        n = self.natoms
+        vs = np.full((n, n), float(self.vdw_scale[3]))
+        qs = np.full((n, n), float(self.q_scale[3]))
+        np.fill_diagonal(vs, 0.0)
+        np.fill_diagonal(qs, 0.0)
+        for d in self.dihedrals:
+            i, j = d[0], d[3]
+            vs[i, j] = vs[j, i] = self.vdw_scale[2]
+            qs[i, j] = qs[j, i] = self.q_scale[2]
+        for a in self.angles:
+            i, j = a[0], a[2]
+            vs[i, j] = vs[j, i] = self.vdw_scale[1]
+            qs[i, j] = qs[j, i] = self.q_scale[1]
+        for b in self.bonds:
+            i, j = b[0], b[1]
+            vs[i, j] = vs[j, i] = self.vdw_scale[0]
+            qs[i, j] = qs[j, i] = self.q_scale[0]
+        return vs, qs
+
+    def bonded_pairs(self):
+        """l_bonded(ia, ja): pairs joined by a bond, angle or dihedral end."""
+        n = self.natoms
+        b = np.zeros((n, n), dtype=bool)
+        for t in list(self.bonds) + [(a[0], a[2]) for a in self.angles] + [(d[0], d[3]) for d in self.dihedrals]:
+            b[t[0], t[1]] = b[t[1], t[0]] = True
+        return b
+
+
+@dataclass
+class System:
+    """One simulation box of the reference (box_list, atom_list, styles)."""
+    species: List[Species]
+    nmols: List[int]
+    coords: List[np.ndarray]          # per species: (nmols, natoms, 3)
+    box: np.ndarray                   # (3,) edge lengths for cubic/ortho
+    temperature: float
+    vdw_style: str = "lj"             # lj | mie | none
+    vdw_sum: str = "cut_tail"
+    rcut_vdw: float = 10.0
+    ron_switch: float = 0.0
+    roff_switch: float = 0.0
+    charge_style: str = "none"        # none | ewald | dsf | cut | minimum_image
+    rcut_coul: float = 0.0
+    ewald_tol: float = 1e-5
+    alpha_dsf: Optional[float] = None
+    rcut_low: float = 1.0
+    rcut_cbmc: float = 6.5
+    mixing: str = "LB"
+    box_shape: str = "cubic"          # cubic | orthogonal
+    seed: tuple = (1234567, 7654321)
+    name: str = "box"
+    max_disp: float = 0.5             # A
+    max_rot: float = 17.0             # degrees, as the input file gives it
+    prob_translate: float = 1.0
+
+    def max_rot_radians(self):
+        """max_rot as the reference holds it: degrees * PI / 180 with the
+        truncated PI (input_routines.f90:4446-4448)."""
+        return self.max_rot * PI / 180.0
+
+    # ------------------------------------------------------------------ types
+    def atom_types(self):
+        """Global atom types in order of first appearance (species, atom), as
+        input_routines.f90 numbers them; returns (type_of[is][ia], type_atoms)."""
+        names, reps, type_of = [], [], []
+        for sp in self.species:
+            t = []
+            for a in sp.atoms:
+                if a.name not in names:
+                    names.append(a.name)
+                    reps.append(a)
+                t.append(names.index(a.name))
+            type_of.append(t)
+        return type_of, reps
+
+    def type_tables(self):
+        """(eps_ij, sig_ij, n_ij, m_ij) in internal units, create_nonbond_table.f90:230-440."""
+        _, reps = self.atom_types()
+        nt = len(reps)
+        eps = np.zeros((nt, nt))
+        sig = np.zeros((nt, nt))
+        rn = np.zeros((nt, nt))
+        rm = np.zeros((nt, nt))
+        e_self = [KBOLTZ * a.eps_K for a in reps]
+        e_self = [0.0 if e <= TINY_NUMBER else e for e in e_self]
+        s_self = [0.0 if a.sigma <= TINY_NUMBER else a.sigma for a in reps]
+        for i in range(nt):
+            for j in range(nt):
+                if i == j:
+                    eps[i, i], sig[i, i] = e_self[i], s_self[i]
+                else:
+                    ei, ej = e_self[i], e_self[j]
+                    eps[i, j] = 0.0 if (ei <= TINY_NUMBER or ej <= TINY_NUMBER) else math.sqrt(ei * ej)
+                    si, sj = s_self[i], s_self[j]
+                    if si <= TINY_NUMBER or sj <= TINY_NUMBER:
+                        sig[i, j] = 0.0
+                    elif self.mixing == "LB":
+                        sig[i, j] = (si + sj) * 0.5
+                    else:
+                        sig[i, j] = math.sqrt(si * sj)
+                # Mie cross terms take the exponents of the lower type index;
+                # the reference requires them to be equal (create_nonbond_table.f90:457-480)
+                rn[i, j] = reps[min(i, j)].mie_n
+                rm[i, j] = reps[min(i, j)].mie_m
+        return eps, sig, rn, rm
+
+    # -------------------------------------------------------------- constants
+    @property
+    def beta(self):
+        return 1.0 / (KBOLTZ * self.temperature)
+
+    @property
+    def volume(self):
+        return float(self.box[0] * self.box[1] * self.box[2])
+
+    def ewald_alpha(self):
+        p = -math.log(self.ewald_tol)
+        return math.sqrt(p) / self.rcut_coul, 2.0 * p / self.rcut_coul
+
+    def dsf_constants(self):
+        """alpha, factor1, factor2 of input_routines.f90:729-757 (A&S erfc)."""
+        rc = self.rcut_coul
+        a = self.alpha_dsf
+        if a is None:
+            a = 0.425 - rc * 0.02
+            if a < 0.0:
+                a = 3.3930702 / rc
+        f1 = erfc_as(a * rc) / rc
+        f2 = f1 / rc + 2.0 * a * math.exp(-a * a * rc * rc) / (ROOT_PI * rc)
+        return a, f1, f2
+
+    def nsites(self):
+        return sum(n * sp.natoms for n, sp in zip(self.nmols, self.species))
+
+    def parsed(self) -> "System":
+        """Copy in which every number that the reference reads from an input
+        or MCF file is replaced by the double its parser produces
+        (io_utilities.f90:331-429); coordinates (read from .xyz frames with
+        list-directed READ) are unchanged."""
+        from .formats import parsed as P
+        sp2 = []
+        for sp in self.species:
+            atoms = [Atom(a.name, a.element, P(a.mass), P(a.charge), P(a.eps_K), P(a.sigma),
+                          P(a.mie_n), P(a.mie_m)) for a in sp.atoms]
+            sp2.append(Species(sp.name, atoms, sp.ref_coords, sp.bonds, sp.angles, sp.dihedrals,
+                               tuple(P(v) for v in sp.vdw_scale), tuple(P(v) for v in sp.q_scale),
+                               sp.fragments, sp.fragment_bonds))
+        s = System(**{**self.__dict__})
+        s.species = sp2
+        s.box = np.array([P(v) for v in self.box])
+        for f in ("temperature", "rcut_vdw", "ron_switch", "roff_switch", "rcut_coul", "ewald_tol",
+                  "rcut_low", "rcut_cbmc", "max_disp", "max_rot", "prob_translate"):
+            setattr(s, f, P(getattr(self, f)))
+        if self.alpha_dsf is not None:
+            s.alpha_dsf = P(self.alpha_dsf)
+        return s
+
+
+def erfc_as(x):
+    """Abramowitz & Stegun 7.1.26 as written at energy_routines.f90:2143-2163."""
+    t = 1.0 / (1.0 + 0.3275911 * x)
+    tp = t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))))
+    return tp * math.exp(-(x * x))
+
+
+# --------------------------------------------------------------------------
+# species library (the molecules named by the benchmark configurations)
+# --------------------------------------------------------------------------
+def species_lj(name="Ar", eps_K=120.0, sigma=3.4, mass=39.948):
+    return Species(name, [Atom(name, name[:2], mass, 0.0, eps_K, sigma)], np.zeros((1, 3)))
+
+
+def species_spce(qO=-0.8476, qH=0.4238, eps_K=78.2, sigma=3.166):
+    th = math.radians(109.47)
+    xyz = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [math.cos(th), math.sin(th), 0.0]])
+    atoms = [Atom("O_w", "O", 15.999, qO, eps_K, sigma),
+             Atom("H_w", "H", 1.008, qH, 0.0, 0.0),
+             Atom("H_w", "H", 1.008, qH, 0.0, 0.0)]
+    return Species("spce", atoms, xyz, bonds=[(0, 1, 1.0), (0, 2, 1.0)],
+                   angles=[(1, 0, 2, "fixed", (109.47,))])
+
+
+def species_methane():
+    return species_lj("CH4", 148.0, 3.73, 16.043)
+
+
+def species_co2():
+    xyz = np.array([[0.0, 0.0, 0.0], [1.16, 0.0, 0.0], [-1.16, 0.0, 0.0]])
+    atoms = [Atom("C_co2", "C", 12.011, 0.70, 27.0, 2.80),
+             Atom("O_co2", "O", 15.999, -0.35, 79.0, 3.05),
+             Atom("O_co2", "O", 15.999, -0.35, 79.0, 3.05)]
+    return Species("co2", atoms, xyz, bonds=[(0, 1, 1.16), (0, 2, 1.16)],
+                   angles=[(1, 0, 2, "fixed", (180.0,))])
+
+
+def species_n2():
+    """3-site N2 with a massless centre charge site."""
+    xyz = np.array([[0.0, 0.0, 0.0], [0.55, 0.0, 0.0], [-0.55, 0.0, 0.0]])
+    atoms = [Atom("N_c", "X", 0.0, 0.964, 0.0, 0.0),
+             Atom("N_n2", "N", 14.007, -0.482, 36.0, 3.31),
+             Atom("N_n2", "N", 14.007, -0.482, 36.0, 3.31)]
+    return Species("n2", atoms, xyz, bonds=[(0, 1, 0.55), (0, 2, 0.55)],
+                   angles=[(1, 0, 2, "fixed", (180.0,))])
+
+
+def species_pentane():
+    """TraPPE-UA n-pentane (CH3-CH2-CH2-CH2-CH3), all-trans reference geometry."""
+    b, th = 1.54, math.radians(114.0)
+    xyz = [np.zeros(3)]
+    d1 = np.array([1.0, 0.0, 0.0])
+    for k in range(1, 5):
+        ang = (math.pi - th) / 2.0 * (1 if k % 2 else -1)
+        d = np.array([math.cos(ang), math.sin(ang), 0.0])
+        xyz.append(xyz[-1] + b * d)
+    xyz = np.array(xyz)
+    ch3 = dict(mass=15.035, charge=0.0, eps_K=98.0, sigma=3.75)
+    ch2 = dict(mass=14.027, charge=0.0, eps_K=46.0, sigma=3.95)
+    atoms = [Atom("CH3", "C", **ch3), Atom("CH2", "C", **ch2), Atom("CH2", "C", **ch2),
+             Atom("CH2", "C", **ch2), Atom("CH3", "C", **ch3)]
+    kang = 31250.0  # K/rad^2 in k(theta-theta0)^2
+    return Species("pentane", atoms, xyz,
+                   bonds=[(i, i + 1, 1.54) for i in range(4)],
+                   angles=[(i, i + 1, i + 2, "harmonic", (kang, 114.0)) for i in range(3)],
+                   dihedrals=[(i, i + 1, i + 2, i + 3, "OPLS", (0.0, 2.95188, -0.566963, 6.57940))
+                              for i in range(2)],
+                   vdw_scale=(0.0, 0.0, 0.0, 1.0), q_scale=(0.0, 0.0, 0.0, 1.0),
+                   fragments=[[0, 1, 2], [1, 2, 3], [2, 3, 4]],
+                   fragment_bonds=[(0, 1), (1, 2)])
+
+
+# --------------------------------------------------------------------------
+# seeded start configurations
+# --------------------------------------------------------------------------
+def random_rotation(rng):
+    """Uniform random rotation matrix from a uniform unit quaternion (Shoemake)."""
+    u1, u2, u3 = rng.random(3)
+    q = np.array([math.sqrt(1 - u1) * math.sin(2 * math.pi * u2),
+                  math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                  math.sqrt(u1) * math.sin(2 * math.pi * u3),
+                  math.sqrt(u1) * math.cos(2 * math.pi * u3)])
+    return quat_to_matrix(q)
+
+
+def quat_to_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def com_of(sp: Species, xyz):
+    m = np.array([a.mass for a in sp.atoms])
+    if m.sum() == 0.0:
+        return xyz.mean(0)
+    return (m[:, None] * xyz).sum(0) / m.sum()
+
+
+def place_random(species_counts, box, seed, dmin_site=1.2, dmin_heavy=2.5, max_tries=2000000):
+    """Uniform random COM positions and uniform orientations with overlap
+    rejection under the minimum image (seeded; not the reference's Make_Config)."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=float)
+    placed_xyz = np.zeros((0, 3))
+    placed_heavy = np.zeros(0, dtype=bool)
+    out = []
+    tries = 0
+    for sp, n in species_counts:
+        ref = sp.ref_coords - com_of(sp, sp.ref_coords)
+        heavy = np.array([a.eps_K > 0.0 for a in sp.atoms])
+        mols = np.zeros((n, sp.natoms, 3))
+        k = 0
+        while k < n:
+            tries += 1
+            if tries > max_tries:
+                raise RuntimeError("could not place molecules; lower the density")
+            c = (rng.random(3) - 0.5) * box
+            R = random_rotation(rng) if sp.natoms > 1 else np.eye(3)
+            xyz = ref @ R.T + c
+            if placed_xyz.shape[0]:
+                d = xyz[:, None, :] - placed_xyz[None, :, :]
+                d -= box * np.round(d / box)
+                r2 = (d * d).sum(-1)
+                if (r2 < dmin_site ** 2).any():
+                    continue
+                hh = heavy[:, None] & placed_heavy[None, :]
+                if (r2[hh] < dmin_heavy ** 2).any():
+                    continue
+            mols[k] = xyz
+            placed_xyz = np.vstack([placed_xyz, xyz])
+            placed_heavy = np.concatenate([placed_heavy, heavy])
+            k += 1
+        out.append(mols)
+    return out
+
+
+# --------------------------------------------------------------------------
+# the benchmark configurations (seeded, frozen here)
+# --------------------------------------------------------------------------
+def lj_box_length(n, rho_star, sigma):
+    return (n / rho_star) ** (1.0 / 3.0) * sigma
+
+
+def config_lj500(seed=20240601):
+    """500 argon-like LJ atoms, rho* = 0.8, T = 120 K, rc = 8.5 A cut_tail."""
+    sp = species_lj()
+    L = lj_box_length(500, 0.8, 3.4)
+    coords = place_random([(sp, 500)], [L] * 3, seed, dmin_site=0.8 * 3.4, dmin_heavy=0.8 * 3.4)
+    return System([sp], [500], coords, np.array([L] * 3), 120.0, "lj", "cut_tail", 8.5,
+                  charge_style="none", rcut_low=1.0, rcut_cbmc=6.5, name="lj500",
+                  max_disp=0.5, prob_translate=1.0, seed=(1234567, 7654321))
+
+
+def config_spce512(seed=20240602):
+    """512 SPC/E-like waters in a 24.83 A box, T = 298 K, LJ 9 A cut_tail,
+    Ewald 9 A with accuracy 1e-5, translate/rotate 50/50."""
+    sp = species_spce()
+    L = 24.83
+    coords = place_random([(sp, 512)], [L] * 3, seed, dmin_site=1.2, dmin_heavy=2.5)
+    return System([sp], [512], coords, np.array([L] * 3), 298.0, "lj", "cut_tail", 9.0,
+                  charge_style="ewald", rcut_coul=9.0, ewald_tol=1e-5, rcut_low=0.85,
+                  rcut_cbmc=6.5, name="spce512", max_disp=0.3, max_rot=17.0,
+                  prob_translate=0.5, seed=(2468013, 1357924))
+
+
+def config_lj2000(seed=20240603):
+    """2000-atom LJ frame at rho* = 0.7, T* = 1.2 (sigma 3.4 A, eps 120 K)."""
+    sp = species_lj()
+    L = lj_box_length(2000, 0.7, 3.4)
+    coords = place_random([(sp, 2000)], [L] * 3, seed, dmin_site=0.85 * 3.4, dmin_heavy=0.85 * 3.4)
+    return System([sp], [2000], coords, np.array([L] * 3), 144.0, "lj", "cut_tail", 8.5,
+                  charge_style="none", rcut_low=1.0, rcut_cbmc=6.5, name="lj2000",
+                  max_disp=0.5, prob_translate=1.0, seed=(97531, 86420))
+
+
+def config_co2n2(seed=20240605):
+    """400 CO2 + 400 N2 in a 40 A box, Ewald 12 A / 1e-5, T = 300 K."""
+    a, b = species_co2(), species_n2()
+    L = 40.0
+    coords = place_random([(a, 400), (b, 400)], [L] * 3, seed, dmin_site=1.5, dmin_heavy=2.8)
+    return System([a, b], [400, 400], coords, np.array([L] * 3), 300.0, "lj", "cut_tail", 12.0,
+                  charge_style="ewald", rcut_coul=12.0, ewald_tol=1e-5, rcut_low=0.8,
+                  rcut_cbmc=6.5, name="co2n2", max_disp=0.5, max_rot=17.0, prob_translate=0.5,
+                  seed=(11223, 44556))
+
+
+def with_probe(system: System, probe: Species):
+    """The same frame with a test species (zero molecules) appended, as the
+    reference's Widom runs declare it."""
+    s = System(**{**system.__dict__})
+    s.species = list(system.species) + [probe]
+    s.nmols = list(system.nmols) + [0]
+    s.coords = list(system.coords) + [np.zeros((0, probe.natoms, 3))]
+    s.name = system.name + "_" + probe.name
+    return s

@@ -1,0 +1,332 @@
+/*
+ * cass_gpu.h -- C ABI of the MI355X energy engine for Cassandra's Monte Carlo
+ * moves (libcassgpu.so).
+ *
+ * Every entry point takes plain scalars and pointers, returns an int status
+ * (CASS_OK = 0) and replaces one Fortran subroutine of the reference; the
+ * replaced routine is cited as file:line into the reference's Src/ tree.
+ * Arguments mirror the Fortran ones: int32 for default INTEGER, double for
+ * REAL(DP), int32 0/1 for LOGICAL.  Indices (box, molecule, species, atom,
+ * type) are 0-based here; the Fortran binding subtracts one.
+ *
+ * Ownership: the host owns the canonical state (coordinates, molecule
+ * lists).  The engine keeps device mirrors and must be told about every
+ * mutation (cass_mol_update after an accepted move or a fold,
+ * cass_box_upload after a frame load or a volume change).
+ *
+ * Threading: one engine is driven by one host thread.  Calls are ordered on
+ * the engine's HIP stream; calls that return values synchronise.
+ */
+#ifndef CASS_GPU_H
+#define CASS_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  CASS_OK = 0,
+  CASS_ERR_ARG = 1,        /* bad argument (index out of range, bad size) */
+  CASS_ERR_HIP = 2,        /* HIP runtime error */
+  CASS_ERR_NOMEM = 3,      /* device allocation failed */
+  CASS_ERR_KVEC = 4,       /* too many k vectors: energy_routines.f90:2268-2272 */
+  CASS_ERR_CHARGE = 5,     /* net charge with Ewald/DSF: main.f90:322-328 */
+  CASS_ERR_STATE = 6,      /* call out of order (e.g. Ewald not set up) */
+  CASS_ERR_NODEVICE = 7,   /* no GPU visible */
+  CASS_ERR_CUTOFF = 8      /* cutoff larger than half the box: input_routines.f90:466-471 */
+};
+
+/* ---------------------------------------------------------------- styles */
+enum { CASS_VDW_NONE = 0, CASS_VDW_LJ = 1, CASS_VDW_MIE = 2 };
+enum {
+  CASS_SUM_CUT = 0, CASS_SUM_CUT_TAIL = 1, CASS_SUM_CUT_SHIFT = 2, CASS_SUM_CUT_SWITCH = 3,
+  CASS_SUM_CUT_SHIFT_FORCE = 4, CASS_SUM_CHARMM = 5, CASS_SUM_MINIMUM = 6
+};
+enum { CASS_Q_NONE = 0, CASS_Q_EWALD = 1, CASS_Q_DSF = 2, CASS_Q_CUT = 3, CASS_Q_MINIMUM = 4 };
+enum { CASS_BOX_CUBIC = 0, CASS_BOX_ORTHO = 1, CASS_BOX_CELL = 2 };
+/* Update_System_Ewald_Reciprocal_Energy move flags (global_variables.f90 int_*) */
+enum { CASS_MOVE_TRANSLATION = 0, CASS_MOVE_ROTATION = 1, CASS_MOVE_INTRA = 2,
+       CASS_MOVE_INSERTION = 3, CASS_MOVE_DELETION = 4 };
+/* arithmetic of the pair and k-space terms */
+enum { CASS_PREC_FP64 = 0,     /* plain FP64 everywhere (reference parity) */
+       CASS_PREC_MIXED = 1 };  /* FP32 pair/k terms, FP64 per-molecule and total sums */
+
+/* --------------------------------------------------------- box parameters
+ * The box geometry and the VDW_Style / Charge_Style / Rcutoff_Low /
+ * CBMC_Info settings of one box (input_routines.f90:425-812, 3784-3830,
+ * 5931-6104, 6941-7104).  The "derived" block holds what the reference
+ * computes from them (compute_cell_dimensions.f90, precalculate.f90:40-90,
+ * input_routines.f90:711-757, 4090): cass_params_derive() fills it from the
+ * raw block; a Fortran host may copy its own module variables instead. */
+typedef struct cass_params {
+  int32_t box_shape;        /* CASS_BOX_* */
+  int32_t vdw_style;        /* CASS_VDW_* */
+  int32_t vdw_sum_style;    /* CASS_SUM_* */
+  int32_t charge_style;     /* CASS_Q_* */
+  double cell[9];           /* box_list%length(3,3), Fortran column-major */
+  double rcut_vdw, rcut_coul, rcut_cbmc, rcut_low;
+  double ron_switch, roff_switch;   /* cut_switch / CHARMM on-off radii */
+  double ewald_tol;                 /* Ewald accuracy */
+  double alpha_dsf_in;              /* DSF damping; <= 0 selects the default */
+  double temperature;               /* K */
+  /* derived */
+  double length_inv[9];     /* box_list%length_inv, column-major */
+  double hlength[3];        /* half basis lengths */
+  double basis_length[3];
+  double volume;
+  double rcut_vdwsq, rcut_coulsq, rcut_cbmcsq, rcut_lowsq, rcut_max;
+  double ron_switch_sq, roff_switch_sq, switch_factor1, switch_factor2;
+  double rcut3, rcut9;
+  double alpha_ewald, h_ewald_cut;
+  double alpha_dsf, dsf_factor1, dsf_factor2;
+  double beta;
+} cass_params;
+
+/* ------------------------------------------------------------- topology
+ * Force field and species shared by all boxes of an engine: the nonbond
+ * tables vdw_param1..4_table (create_nonbond_table.f90:230-440, in internal
+ * units), per-atom type/charge/mass (nonbond_list), the intramolecular
+ * scaling tables (create_intra_exclusion_table.f90) and l_bonded. */
+typedef struct cass_topology {
+  int32_t ntypes;
+  const double *eps;        /* [ntypes*ntypes] vdw_param1_table */
+  const double *sig;        /* [ntypes*ntypes] vdw_param2_table */
+  const double *mie_n;      /* [ntypes*ntypes] vdw_param3_table (NULL for LJ) */
+  const double *mie_m;      /* [ntypes*ntypes] vdw_param4_table (NULL for LJ) */
+  int32_t nspecies;
+  const int32_t *species_natoms;      /* [nspecies] */
+  const int32_t *species_atom_offset; /* [nspecies] into the per-atom arrays */
+  const int32_t *species_pair_offset; /* [nspecies] into the natoms^2 tables */
+  const int32_t *species_coul_cbmc;   /* [nspecies] L_Coul_CBMC */
+  const int32_t *atom_type;           /* [sum natoms] 0-based type, -1 for none */
+  const double *atom_charge;          /* [sum natoms] */
+  const double *atom_mass;            /* [sum natoms] */
+  const double *vdw_intra_scale;      /* [sum natoms^2] row-major (ia, ja) */
+  const double *charge_intra_scale;   /* [sum natoms^2] */
+  const int32_t *bonded;              /* [sum natoms^2] l_bonded */
+} cass_topology;
+
+/* Energy_Class of one box (type_definitions.f90:487-511), internal units. */
+typedef struct cass_energy {
+  double total, intra, inter;
+  double bond, angle, dihedral, improper, intra_vdw, intra_q;
+  double inter_vdw, lrc, inter_q, reciprocal, self;
+} cass_energy;
+
+typedef struct cass_engine cass_engine;
+
+/* ------------------------------------------------------------------ setup */
+/* Fills the derived block of *p (compute_cell_dimensions.f90:50-170,
+ * precalculate.f90:40-90, input_routines.f90:711-757 and 4090). */
+int cass_params_derive(cass_params *p);
+
+/* Creates an engine on HIP device `device` for up to `max_boxes` boxes.
+ * The topology is copied. */
+int cass_engine_create(int32_t device, const cass_topology *topo, int32_t max_boxes,
+                       cass_engine **out);
+int cass_engine_destroy(cass_engine *eng);
+int cass_set_precision(cass_engine *eng, int32_t prec);   /* CASS_PREC_* */
+const char *cass_last_error(cass_engine *eng);
+/* Device time of the last batched kernel (Widom / trial-set), from HIP events
+ * recorded on the engine's stream around the launch.  Instrumentation; no
+ * counterpart in the reference (whose timers are the CPU clocks of
+ * nvtmc_driver.f90:110-232 and widom_insert.f90:322-323). */
+int cass_last_kernel_ms(cass_engine *eng, double *ms);
+
+/* Adds a box able to hold `max_mols` molecules; returns its id in *box. */
+int cass_box_create(cass_engine *eng, const cass_params *p, int32_t max_mols, int32_t *box);
+
+/* Mirrors atom_list / molecule_list of a box: nmols molecules of
+ * mol_species[m], live flag live[m], coordinates xyz[3*site] (sites of
+ * molecule m follow those of m-1), COM com[3*m] and max_dcom[m]
+ * (Get_COM / Compute_Max_Com_Distance, get_com.f90:45-121,220-295).  com and
+ * max_dcom may be NULL: the engine then computes them as the reference does. */
+int cass_box_upload(cass_engine *eng, int32_t box, int32_t nmols, const int32_t *mol_species,
+                    const int32_t *live, const double *xyz, const double *com,
+                    const double *max_dcom);
+
+/* New coordinates of one molecule (accepted move, fold, insertion);
+ * live sets molecule_list%live. com/max_dcom as in cass_box_upload. */
+int cass_mol_update(cass_engine *eng, int32_t box, int32_t mol, int32_t live,
+                    const double *xyz, const double *com, const double *max_dcom);
+
+/* Reads back the device copy of the coordinates of a box (3*nsites). */
+int cass_box_download(cass_engine *eng, int32_t box, double *xyz);
+
+/* --------------------------------------------------------- pair energies */
+/* Compute_Molecule_Nonbond_Inter_Energy (energy_routines.f90:1088-1206):
+ * molecule `mol` of `box` against every other live molecule, with the COM
+ * prefilter of Check_MoleculePair_Cutoff (:3027-3088), the overlap test
+ * r^2 < rcut_low^2 (:1676) and the style dispatch of Compute_AtomPair_Energy
+ * (:1831-2053).  If xyz is non-NULL the molecule is evaluated at those
+ * coordinates (com / max_dcom as in cass_box_upload) instead of its stored
+ * ones, which is how a trial position is scored before commit.  cbmc_flag
+ * selects the CBMC cutoff (rcut_cbmc, L_Coul_CBMC). */
+int cass_mol_inter_energy(cass_engine *eng, int32_t box, int32_t mol, int32_t cbmc_flag,
+                          const double *xyz, const double *com, const double *max_dcom,
+                          double *e_vdw, double *e_qq, int32_t *overlap);
+
+/* Compute_Molecule_Nonbond_Intra_Energy (energy_routines.f90:980-1085) for a
+ * molecule of `species` at xyz (natoms*3): minimum-image intramolecular vdW
+ * and q-q with 1-2/1-3/1-4/1-N scaling, the periodic part of Ewald/DSF as
+ * e_inter_qq, and the intra overlap test r^2 <= rcut_low^2 for non-bonded
+ * pairs.  Host arithmetic (n_m^2 <= 25 pairs). */
+int cass_mol_intra_energy(cass_engine *eng, int32_t box, int32_t species, int32_t cbmc_flag,
+                          const double *xyz, double *e_intra_vdw, double *e_intra_qq,
+                          double *e_inter_qq, int32_t *overlap);
+
+/* Trial energies of a CBMC / Widom trial set in one launch: ntrial
+ * placements of a molecule of `species` (trial_xyz[ntrial*natoms*3]) against
+ * every live molecule of the box except `exclude_mol` (-1: none).  Each trial
+ * is the sum Compute_Molecule_Nonbond_Inter_Energy[_Widom] returns
+ * (energy_routines.f90:1088,1209) at cutoff rcut_cbmc when cbmc_flag is set
+ * (fragment_growth.f90:519-526) and at the full cutoff otherwise; nrg[t] is
+ * E_vdw + E_qq, overlap[t] the overlap flag.  Replaces the serial kappa loops
+ * of Build_Molecule (fragment_growth.f90:383-579) and Fragment_Placement
+ * (:1926-2058). */
+int cass_trial_energies(cass_engine *eng, int32_t box, int32_t species, int32_t ntrial,
+                        const double *trial_xyz, int32_t cbmc_flag, int32_t exclude_mol,
+                        double *nrg, int32_t *overlap);
+
+/* Same, for the trial sets of several boxes in one persistent launch:
+ * set s has ntrial[s] trials in box box_of_set[s], species species_of_set[s],
+ * coordinates at trial_xyz + 3*xyz_offset[s], results at nrg/overlap +
+ * out_offset[s]. */
+int cass_trial_energies_multi(cass_engine *eng, int32_t nsets, const int32_t *box_of_set,
+                              const int32_t *species_of_set, const int32_t *ntrial,
+                              const int64_t *xyz_offset, const int64_t *out_offset,
+                              const double *trial_xyz, int32_t cbmc_flag,
+                              const int32_t *exclude_mol, double *nrg, int32_t *overlap);
+
+/* Compute_System_Total_Energy (energy_routines.f90:2648-2953) with intra
+ * terms restricted to the nonbonded part (bonded terms stay on the host):
+ * fills *e and *overlap. */
+int cass_total_energy(cass_engine *eng, int32_t box, int32_t intra_flag, cass_energy *e,
+                      int32_t *overlap);
+
+/* Compute_LR_Correction (energy_routines.f90:2956-3023) for the given bead
+ * counts per type (nint_beads[ntypes]). */
+int cass_lrc(cass_engine *eng, int32_t box, const double *nint_beads, double *e_lrc);
+
+/* Compute_Molecule_Self_Energy (energy_routines.f90:2598-2645). */
+int cass_mol_self_energy(cass_engine *eng, int32_t box, int32_t species, double *e_self);
+
+/* ------------------------------------------------------------ reciprocal */
+/* Ewald_Reciprocal_Lattice_Vector_Setup (energy_routines.f90:2170-2323):
+ * same k set, order (nz, ny, nx) and Cn.  Returns the count in *nvecs. */
+int cass_ewald_setup(cass_engine *eng, int32_t box, int32_t *nvecs);
+
+/* Compute_System_Ewald_Reciprocal_Energy (energy_routines.f90:4111-4231):
+ * per-molecule cos/sin sums, box sums and the reciprocal energy. */
+int cass_ewald_full(cass_engine *eng, int32_t box, double *e_recip);
+
+/* Update_System_Ewald_Reciprocal_Energy (energy_routines.f90:2326-2497) for
+ * molecule `mol` at new coordinates xyz (translation / rotation / intra /
+ * insertion; NULL for deletion).  The box sums are updated tentatively and
+ * the new TOTAL reciprocal energy is returned, as the reference does;
+ * *de_recip (may be NULL) gets E_new - E_old computed term by term
+ * (sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2)), which keeps its relative
+ * accuracy in mixed precision.  Follow with cass_ewald_commit or
+ * cass_ewald_revert (the caller restore of move_translate.f90:296-306). */
+int cass_ewald_move(cass_engine *eng, int32_t box, int32_t mol, int32_t move_flag,
+                    const double *xyz, double *e_recip_total, double *de_recip);
+int cass_ewald_commit(cass_engine *eng, int32_t box);
+int cass_ewald_revert(cass_engine *eng, int32_t box);
+
+/* Update_System_Ewald_Reciprocal_Energy_Widom (energy_routines.f90:2500-2550)
+ * for ntrial test molecules of `species` at trial_xyz, without mutating the
+ * box sums: e_recip[t] is the total reciprocal energy with molecule t added. */
+int cass_ewald_widom(cass_engine *eng, int32_t box, int32_t species, int32_t ntrial,
+                     const double *trial_xyz, double *e_recip);
+
+/* ----------------------------------------------------------------- Widom */
+/* The insertion loop of Widom_Insert (widom_insert.f90:319-489) for a rigid,
+ * single-fragment test species with kappa_ins = 1, all insertions in one
+ * launch.  Placement u[0..5] per insertion comes from a counter-based stream
+ * (seed, insertion index): COM = (u0,u1,u2) - 1/2 in box units and a uniform
+ * rotation from (u3,u4,u5).  Per insertion: full-cutoff inter energy, the
+ * intramolecular periodic Ewald term, the reciprocal energy with the test
+ * molecule added, and widom_var = exp(-beta*dE) with
+ * dE = E_inter + e_const (e_const = lrc_diff + E_self - E_recip_in + intra).
+ * Sums are FP64.  If widom_var is non-NULL it receives every insertion's
+ * value (n_ins doubles). */
+typedef struct cass_widom_result {
+  double widom_sum;         /* sum of widom_var (prefactor V/(N Lambda^3) NOT applied) */
+  int64_t n_overlaps;       /* insertions rejected by r^2 < rcut_low^2 */
+  int64_t n_insertions;
+  double e_const;           /* lrc_diff + E_self - E_recip_in + intramolecular */
+} cass_widom_result;
+
+int cass_widom_batch(cass_engine *eng, int32_t box, int32_t species, int64_t n_ins,
+                     uint64_t seed, double *widom_var, cass_widom_result *res);
+
+/* Rigid geometry of the test species (xyz[natoms*3], any position and
+ * orientation) used by cass_widom_batch; *e_intra_const receives its
+ * intramolecular nonbonded energy (Compute_Molecule_Nonbond_Intra_Energy,
+ * energy_routines.f90:980-1085: intra vdW + intra q-q + periodic q-q), which
+ * is constant for a rigid molecule and enters widom_var through e_const. */
+int cass_widom_set_template(cass_engine *eng, int32_t box, int32_t species, const double *xyz,
+                            double *e_intra_const);
+
+/* The per-insertion body of Widom_Insert (widom_insert.f90:378-447) at n
+ * explicit placements xyz[n*natoms*3], through the same kernel as
+ * cass_widom_batch: eout[4*i..] = {E_inter at the full cutoff, E_inter at
+ * rcut_cbmc, reciprocal energy change, overlap}; widom_var[i] as in
+ * cass_widom_batch (either pointer may be NULL). */
+int cass_widom_placements(cass_engine *eng, int32_t box, int32_t species, int64_t n,
+                          const double *xyz, double *widom_var, double *eout,
+                          cass_widom_result *res);
+
+/* ---------------------------------------------------------------- adjoint
+ * New (no counterpart in the reference): derivatives of the box's total
+ * intermolecular pair energy (inter_vdw + inter_q, real-space Ewald/DSF/cut)
+ * plus the reciprocal and self energies with respect to the per-type LJ
+ * parameters eps[t] and sigma[t] of the diagonal table entries (propagated
+ * through Lorentz-Berthelot mixing sigma_ij = (sigma_i+sigma_j)/2,
+ * eps_ij = sqrt(eps_i eps_j)) and the per-atom-kind charges, plus forces
+ * -dE/dr per site.  Accumulated in FP64.  Arrays: de_deps[ntypes],
+ * de_dsig[ntypes], de_dq[sum natoms] (per species atom), forces[3*nsites]. */
+int cass_adjoint(cass_engine *eng, int32_t box, double *energy, double *de_deps,
+                 double *de_dsig, double *de_dq, double *forces);
+
+/* -------------------------------------------------------- host-side chain
+ * The reference's random-number stream (random_generators.f90:51-122), the
+ * Metropolis test (accept_or_reject.f90:22-65) and the NVT move loop of
+ * nvtmc_driver.f90:94-152 with Translate (move_translate.f90:23) and Rotate
+ * (move_rotate.f90:23) for one box, every energy evaluated by the engine.
+ * Per-move record: kind (0 translate, 1 rotate), accepted flag and the
+ * running total energy after the move. */
+typedef struct cass_rng { int64_t s[5]; } cass_rng;
+void cass_rng_init(cass_rng *r, int64_t i1, int64_t i3);      /* init_seeds, one thread */
+double cass_rng_uniform(cass_rng *r);                         /* rranf */
+int64_t cass_rng_int(cass_rng *r);                            /* rranint */
+int32_t cass_accept_or_reject(cass_rng *r, double ln_pacc);   /* accept_or_reject */
+
+typedef struct cass_nvt_options {
+  double prob_translate;    /* cut_trans; rotation takes the rest */
+  double max_disp;          /* max_disp(is, box) */
+  double max_rot;           /* max_rot(is, box) */
+  int32_t linear;           /* species is linear: Euler rotation (move_rotate.f90:211) */
+  int32_t species;          /* the single mobile species */
+} cass_nvt_options;
+
+typedef struct cass_nvt_record {
+  uint8_t *kind;            /* [nsteps] or NULL */
+  uint8_t *accepted;        /* [nsteps] or NULL */
+  double *energy;           /* [nsteps] running total, or NULL */
+  int64_t *rng_state;       /* [5*nsteps] state after each move, or NULL */
+  double *ln_pacc;          /* [nsteps] beta*dE of moves that reached the test, else NaN */
+} cass_nvt_record;
+
+/* Runs nsteps moves from the box's current state; *energy holds the
+ * Energy_Class on entry (e.g. from cass_total_energy) and is updated. */
+int cass_nvt_run(cass_engine *eng, int32_t box, cass_rng *rng, const cass_nvt_options *opt,
+                 int64_t nsteps, cass_energy *energy, cass_nvt_record *rec);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CASS_GPU_H */

@@ -1,0 +1,1007 @@
+/*
+ * cass_oracle.c -- CPU FP64 restatement of Cassandra's energy path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the checker the HIP engine is compared
+ * against (tests/, __graft_entry__.smoke(), bench.py's cpu_baseline leg); the
+ * product never links or calls it.  Each function cites the reference routine
+ * it restates (file:line into the reference's Src/ tree).  It is compiled with
+ * -ffp-contract=off so every multiply-add rounds twice, as the reference's
+ * x86-64 build does, and follows the reference's operation order where that
+ * is cheap to do.
+ *
+ * Parity status of each function is recorded in oracle/README.md.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/cass_gpu.h"
+
+#define PI_T 3.1415926536          /* global_variables.f90:182 (truncated) */
+#define TWOPI_T 6.2831853072       /* :183 */
+#define ROOTPI_T 1.7724538509      /* :184 */
+#define KBOLTZ 0.8314472           /* :187 */
+#define CHARGE_FACTOR 138935.4558  /* :198 */
+#define TINY_NUMBER 0.0000001      /* :222 */
+#define MAX_KBT 708.0              /* :228 */
+#define MAXK 100000                /* global_variables.f90:443 */
+
+/* ------------------------------------------------------------------ state */
+typedef struct orc_box {
+  cass_params p;
+  cass_topology t;
+  int nmols, nsites;
+  int *mol_species, *mol_live, *mol_first;
+  double *x, *y, *z;            /* site coordinates */
+  double *com, *max_dcom;       /* 3*nmols, nmols */
+  /* Ewald */
+  int nvecs;
+  double *hx, *hy, *hz, *hsq, *cn;
+  double *cs, *sn, *cs_old, *sn_old;
+  double *cmol, *smol;          /* nmols*nvecs, molecule-major */
+  double *cmol_save, *smol_save;
+  int saved_mol;
+} orc_box;
+
+/* -------------------------------------------------- params (derived block) */
+/* compute_cell_dimensions.f90:50-170, precalculate.f90:40-90,
+ * input_routines.f90:711-757 (Ewald/DSF), :4090 (beta) */
+static double erfc_as(double x) {
+  /* energy_routines.f90:2143-2163 (A&S 7.1.26) */
+  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027,
+               A5 = 1.061405429, P = 0.3275911;
+  double T = 1.0 / (1.0 + P * x);
+  double xsq = x * x;
+  double TP = T * (A1 + T * (A2 + T * (A3 + T * (A4 + T * A5))));
+  return TP * exp(-xsq);
+}
+
+int orc_params_derive(cass_params *p) {
+  const double *L = p->cell; /* column-major: L[i + 3*j] = length(i+1, j+1) */
+#define LL(i, j) L[(i - 1) + 3 * (j - 1)]
+  double *Li = p->length_inv;
+#define LI(i, j) Li[(i - 1) + 3 * (j - 1)]
+  for (int j = 1; j <= 3; ++j)
+    p->basis_length[j - 1] = sqrt(LL(1, j) * LL(1, j) + LL(2, j) * LL(2, j) + LL(3, j) * LL(3, j));
+  double bxc1 = LL(2, 2) * LL(3, 3) - LL(3, 2) * LL(2, 3);
+  double bxc2 = LL(3, 2) * LL(1, 3) - LL(1, 2) * LL(3, 3);
+  double bxc3 = LL(1, 2) * LL(2, 3) - LL(2, 2) * LL(1, 3);
+  p->volume = fabs(LL(1, 1) * bxc1 + LL(2, 1) * bxc2 + LL(3, 1) * bxc3);
+  LI(1, 1) = LL(2, 2) * LL(3, 3) - LL(2, 3) * LL(3, 2);
+  LI(2, 1) = LL(3, 1) * LL(2, 3) - LL(2, 1) * LL(3, 3);
+  LI(3, 1) = LL(2, 1) * LL(3, 2) - LL(2, 2) * LL(3, 1);
+  LI(1, 2) = LL(3, 2) * LL(1, 3) - LL(1, 2) * LL(3, 3);
+  LI(2, 2) = LL(1, 1) * LL(3, 3) - LL(3, 1) * LL(1, 3);
+  LI(3, 2) = LL(3, 1) * LL(1, 2) - LL(1, 1) * LL(3, 2);
+  LI(1, 3) = LL(1, 2) * LL(2, 3) - LL(2, 2) * LL(1, 3);
+  LI(2, 3) = LL(2, 1) * LL(1, 3) - LL(1, 1) * LL(2, 3);
+  LI(3, 3) = LL(1, 1) * LL(2, 2) - LL(2, 1) * LL(1, 2);
+  double det = LL(1, 1) * LI(1, 1) + LL(1, 2) * LI(2, 1) + LL(1, 3) * LI(3, 1);
+  double inv_det = 1.0 / det;
+  for (int k = 0; k < 9; ++k) Li[k] = Li[k] * inv_det;
+  for (int k = 0; k < 3; ++k) p->hlength[k] = 0.5 * p->basis_length[k];
+#undef LL
+#undef LI
+  if (p->vdw_sum_style == CASS_SUM_MINIMUM) {
+    double fd = p->volume / sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+    p->rcut_vdw = 0.5 * fd;
+  }
+  if (p->vdw_sum_style == CASS_SUM_CUT_SWITCH) {
+    p->ron_switch_sq = p->ron_switch * p->ron_switch;
+    p->roff_switch_sq = p->roff_switch * p->roff_switch;
+    double d = p->roff_switch_sq - p->ron_switch_sq;
+    p->switch_factor1 = 1.0 / pow(d, 3.0);
+    p->switch_factor2 = p->roff_switch_sq - 3.0 * p->ron_switch_sq;
+    p->rcut_vdw = p->roff_switch;
+    p->rcut_vdwsq = 0.0;
+  } else if (p->vdw_sum_style == CASS_SUM_CHARMM) {
+    p->rcut_vdw = 0.0; /* CHARMM leaves rcut_vdw unset (input_routines.f90:445-458) */
+    p->rcut_vdwsq = 0.0;
+  } else {
+    p->rcut_vdwsq = p->rcut_vdw * p->rcut_vdw;
+  }
+  p->rcut3 = p->rcut_vdw * p->rcut_vdw * p->rcut_vdw;
+  p->rcut9 = p->rcut3 * p->rcut3 * p->rcut3;
+  if (p->charge_style == CASS_Q_MINIMUM) {
+    double fd = p->volume / sqrt(bxc1 * bxc1 + bxc2 * bxc2 + bxc3 * bxc3);
+    p->rcut_coul = 0.5 * fd;
+  }
+  if (p->charge_style != CASS_Q_NONE) p->rcut_coulsq = p->rcut_coul * p->rcut_coul;
+  if (p->vdw_style != CASS_VDW_NONE && p->charge_style != CASS_Q_NONE)
+    p->rcut_max = p->rcut_vdw > p->rcut_coul ? p->rcut_vdw : p->rcut_coul;
+  else if (p->vdw_style != CASS_VDW_NONE)
+    p->rcut_max = p->rcut_vdw;
+  else
+    p->rcut_max = p->rcut_coul;
+  p->rcut_cbmcsq = p->rcut_cbmc * p->rcut_cbmc;
+  p->rcut_lowsq = p->rcut_low * p->rcut_low;
+  if (p->charge_style == CASS_Q_EWALD) {
+    double ep = -log(p->ewald_tol);
+    p->alpha_ewald = sqrt(ep) / p->rcut_coul;
+    p->h_ewald_cut = 2.0 * ep / p->rcut_coul;
+  }
+  if (p->charge_style == CASS_Q_DSF) {
+    double a = p->alpha_dsf_in;
+    if (!(a > 0.0)) {
+      a = 0.425 - p->rcut_coul * 0.02;
+      if (a < 0.0) a = 3.3930702 / p->rcut_coul;
+    }
+    p->alpha_dsf = a;
+    double rc = p->rcut_coul;
+    p->dsf_factor1 = erfc_as(a * rc) / rc;
+    p->dsf_factor2 = p->dsf_factor1 / rc + 2.0 * a * exp(-a * a * rc * rc) / (ROOTPI_T * rc);
+  }
+  p->beta = 1.0 / (KBOLTZ * p->temperature);
+  return 0;
+}
+
+/* --------------------------------------------------------------- geometry */
+static void min_image(const cass_params *p, double dx, double dy, double dz, double *ox,
+                      double *oy, double *oz) {
+  /* minimum_image_separation.f90:35-112 */
+  if (p->box_shape == CASS_BOX_CUBIC) {
+    const double L1 = p->cell[0], L2 = p->cell[4], L3 = p->cell[8];
+    const double *h = p->hlength;
+    double rx = dx, ry = dy, rz = dz;
+    if (dx > h[0]) rx = dx - L1; else if (dx < -h[0]) rx = dx + L1;
+    if (dy > h[1]) ry = dy - L2; else if (dy < -h[1]) ry = dy + L2;
+    if (dz > h[2]) rz = dz - L3; else if (dz < -h[2]) rz = dz + L3;
+    *ox = rx; *oy = ry; *oz = rz;
+  } else {
+    const double *Li = p->length_inv, *L = p->cell;
+    double sx = Li[0] * dx + Li[3] * dy + Li[6] * dz;
+    double sy = Li[1] * dx + Li[4] * dy + Li[7] * dz;
+    double sz = Li[2] * dx + Li[5] * dy + Li[8] * dz;
+    sx = sx - round(sx);
+    sy = sy - round(sy);
+    sz = sz - round(sz);
+    *ox = L[0] * sx + L[3] * sy + L[6] * sz;
+    *oy = L[1] * sx + L[4] * sy + L[7] * sz;
+    *oz = L[2] * sx + L[5] * sy + L[8] * sz;
+  }
+}
+
+/* ------------------------------------------------------------- pair terms */
+static void atompair_cutoff(const cass_params *p, int cbmc, double r2, int *gv, int *gq) {
+  /* Check_AtomPair_Cutoff, energy_routines.f90:3092-3195 */
+  *gv = 0;
+  *gq = 0;
+  if (p->vdw_style == CASS_VDW_LJ) {
+    if (cbmc) *gv = r2 <= p->rcut_cbmcsq;
+    else switch (p->vdw_sum_style) {
+      case CASS_SUM_CUT: case CASS_SUM_CUT_SHIFT: case CASS_SUM_CUT_TAIL: case CASS_SUM_CUT_SHIFT_FORCE:
+        *gv = r2 <= p->rcut_vdwsq; break;
+      case CASS_SUM_MINIMUM: case CASS_SUM_CHARMM: *gv = 1; break;
+      case CASS_SUM_CUT_SWITCH: *gv = r2 <= p->roff_switch_sq; break;
+    }
+  } else if (p->vdw_style == CASS_VDW_MIE) {
+    if (cbmc) *gv = r2 <= p->rcut_cbmcsq;
+    else if (p->vdw_sum_style == CASS_SUM_CUT || p->vdw_sum_style == CASS_SUM_CUT_SHIFT ||
+             p->vdw_sum_style == CASS_SUM_CUT_TAIL)
+      *gv = r2 <= p->rcut_vdwsq;
+  }
+  if (p->charge_style == CASS_Q_CUT || p->charge_style == CASS_Q_EWALD || p->charge_style == CASS_Q_DSF)
+    *gq = cbmc ? (r2 <= p->rcut_cbmcsq) : (r2 <= p->rcut_coulsq);
+  else if (p->charge_style == CASS_Q_MINIMUM)
+    *gq = 1;
+}
+
+/* Compute_AtomPair_Energy (energy_routines.f90:1831-2053) with
+ * Compute_AtomPair_Ewald_Real (:2088-2166) and Compute_AtomPair_DSF_Energy
+ * (:2056-2082).  out = {intra_vdw, intra_qq, inter_vdw, inter_qq}. */
+static void atompair_energy(const orc_box *b, double r2, int ti, int tj, double qi, double qj,
+                            int same_mol, double vscale, double qscale, int gv, int gq,
+                            double out[4]) {
+  const cass_params *p = &b->p;
+  const cass_topology *t = &b->t;
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  if (gv && ti >= 0 && tj >= 0) {
+    double Eij = 0.0;
+    int k = ti * t->ntypes + tj;
+    if (p->vdw_style == CASS_VDW_LJ) {
+      double eps = t->eps[k], sig = t->sig[k];
+      if (same_mol) eps = eps * vscale;
+      double s2 = (sig * sig) / r2;
+      double s6 = s2 * s2 * s2;
+      double s12 = s6 * s6;
+      Eij = 4.0 * eps * (s12 - s6);
+      if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+        double c2 = sig * sig / p->rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        Eij = Eij - 4.0 * eps * (c12 - c6);
+      } else if (p->vdw_sum_style == CASS_SUM_CUT_SWITCH) {
+        if (r2 < p->ron_switch_sq) {
+        } else if (r2 <= p->roff_switch_sq) {
+          double d = p->roff_switch_sq - r2;
+          double d2 = d * d;
+          double f2 = p->switch_factor2 + 2.0 * r2;
+          double fs = d2 * f2 * p->switch_factor1;
+          Eij = fs * Eij;
+        } else {
+          Eij = 0.0;
+        }
+      } else if (p->vdw_sum_style == CASS_SUM_CHARMM) {
+        Eij = eps * (s12 - 2.0 * s6);
+      } else if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT_FORCE) {
+        double c2 = sig * sig / p->rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        Eij = Eij - 4.0 * eps * (c12 - c6);
+        double rij = sqrt(r2), rc = sqrt(p->rcut_vdwsq);
+        double dE = -24.0 * eps * (2.0 * c12 / rc - c6 / rc);
+        Eij = Eij - (rij - rc) * dE;
+      }
+    } else if (p->vdw_style == CASS_VDW_MIE) {
+      double eps = t->eps[k], sig = t->sig[k], n = t->mie_n[k], m = t->mie_m[k];
+      if (same_mol) eps = eps * vscale;
+      double rij = sqrt(r2), rc = sqrt(p->rcut_vdwsq);
+      double coeff = n / (n - m) * pow(n / m, m / (n - m));
+      double sr = sig / rij;
+      Eij = coeff * eps * (pow(sr, n) - pow(sr, m));
+      if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+        double sc = sig / rc;
+        Eij = Eij - coeff * eps * (pow(sc, n) - pow(sc, m));
+      }
+    }
+    if (same_mol) out[0] = Eij; else out[2] = Eij;
+  }
+  if (gq) {
+    if (p->charge_style == CASS_Q_EWALD) {
+      double rij = sqrt(r2);
+      double erf_val = 1.0 - erfc_as(p->alpha_ewald * rij);
+      double Eqq = qi * qj / rij * CHARGE_FACTOR;
+      double intra = 0.0, inter = 0.0;
+      if (same_mol) intra = qscale * Eqq; else inter = Eqq;
+      inter = inter - erf_val * Eqq;
+      out[1] = intra;
+      out[3] = inter;
+    } else if (p->charge_style == CASS_Q_DSF) {
+      double rij = sqrt(r2);
+      double cfqq = qi * qj * CHARGE_FACTOR;
+      double Eij = p->dsf_factor2 * (rij - p->rcut_coul) - p->dsf_factor1 + erfc(p->alpha_dsf * rij) / rij;
+      Eij = Eij * cfqq;
+      if (same_mol) {
+        double Eqq = cfqq / rij;
+        out[1] = Eij - (1.0 - qscale) * Eqq;
+      } else {
+        out[3] = Eij;
+      }
+    } else if (p->charge_style == CASS_Q_CUT || p->charge_style == CASS_Q_MINIMUM) {
+      double Eqq = CHARGE_FACTOR * (qi * qj) / sqrt(r2);
+      if (same_mol) out[1] = qscale * Eqq; else out[3] = Eqq;
+    }
+  }
+}
+
+/* --------------------------------------------------------------- molecules */
+static int species_has_charge(const cass_topology *t, int s) {
+  int o = t->species_atom_offset[s];
+  for (int a = 0; a < t->species_natoms[s]; ++a)
+    if (t->atom_charge[o + a] != 0.0) return 1;
+  return 0;
+}
+
+static void get_com(const orc_box *b, int s, const double *x, const double *y, const double *z,
+                    double com[3], double *max_dcom) {
+  /* Get_COM get_com.f90:45-121; Compute_Max_Com_Distance :220-295 */
+  const cass_topology *t = &b->t;
+  int o = t->species_atom_offset[s], n = t->species_natoms[s];
+  double tm = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
+  for (int a = 0; a < n; ++a) {
+    double m = t->atom_mass[o + a];
+    tm = tm + m;
+    cx = cx + m * x[a];
+    cy = cy + m * y[a];
+    cz = cz + m * z[a];
+  }
+  com[0] = cx / tm;
+  com[1] = cy / tm;
+  com[2] = cz / tm;
+  if (max_dcom) {
+    double dmax = 0.0;
+    for (int a = 0; a < n; ++a) {
+      double dx = x[a] - com[0], dy = y[a] - com[1], dz = z[a] - com[2];
+      double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 > dmax) dmax = r2;
+    }
+    *max_dcom = sqrt(dmax);
+  }
+}
+
+/* Check_MoleculePair_Cutoff (energy_routines.f90:3027-3088) */
+static int molpair_cutoff(const orc_box *b, int cbmc, const double *ci, double dci, const double *cj,
+                          double dcj) {
+  const cass_params *p = &b->p;
+  if (p->vdw_sum_style == CASS_SUM_MINIMUM) return 1;
+  double rx, ry, rz;
+  min_image(p, ci[0] - cj[0], ci[1] - cj[1], ci[2] - cj[2], &rx, &ry, &rz);
+  double rcom = sqrt(rx * rx + ry * ry + rz * rz);
+  double rint = (cbmc ? p->rcut_cbmc : p->rcut_max) + dci + dcj;
+  return !(rcom > rint);
+}
+
+/* Compute_MoleculePair_Energy (energy_routines.f90:1589-1730) for molecule
+ * i given by coordinates (xi,yi,zi) of species si against stored molecule j. */
+static int molpair_energy(const orc_box *b, int cbmc, int si, const double *xi, const double *yi,
+                          const double *zi, int mj, double *vlj, double *vqq) {
+  const cass_topology *t = &b->t;
+  int sj = b->mol_species[mj];
+  int oi = t->species_atom_offset[si], oj = t->species_atom_offset[sj];
+  int ni = t->species_natoms[si], nj = t->species_natoms[sj];
+  int fj = b->mol_first[mj];
+  *vlj = 0.0;
+  *vqq = 0.0;
+  for (int ia = 0; ia < ni; ++ia) {
+    for (int ja = 0; ja < nj; ++ja) {
+      double rx, ry, rz;
+      min_image(&b->p, xi[ia] - b->x[fj + ja], yi[ia] - b->y[fj + ja], zi[ia] - b->z[fj + ja], &rx,
+                &ry, &rz);
+      double r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 < b->p.rcut_lowsq) return 1;
+      int gv, gq;
+      atompair_cutoff(&b->p, cbmc, r2, &gv, &gq);
+      if (cbmc && !t->species_coul_cbmc[si]) gq = 0;
+      if (gv || gq) {
+        double e[4];
+        atompair_energy(b, r2, t->atom_type[oi + ia], t->atom_type[oj + ja], t->atom_charge[oi + ia],
+                        t->atom_charge[oj + ja], 0, 0.0, 0.0, gv, gq, e);
+        *vlj = *vlj + e[2];
+        *vqq = *vqq + e[3];
+      }
+    }
+  }
+  return 0;
+}
+
+/* Compute_Molecule_Nonbond_Inter_Energy (energy_routines.f90:1088-1206) for
+ * a molecule of species si at (x,y,z) with COM com / max_dcom, skipping
+ * stored molecule `self` (-1 none).  combine != 0 sums vdW + qq per molecule
+ * pair first, as the _Widom variant does (:1209-1364). */
+static int mol_inter(const orc_box *b, int cbmc, int si, const double *x, const double *y,
+                     const double *z, const double *com, double dcom, int self, int combine,
+                     double *evdw, double *eqq) {
+  *evdw = 0.0;
+  *eqq = 0.0;
+  for (int s = 0; s < b->t.nspecies; ++s) {
+    for (int m = 0; m < b->nmols; ++m) {
+      if (b->mol_species[m] != s) continue;
+      if (!b->mol_live[m] || m == self) continue;
+      if (!molpair_cutoff(b, cbmc, com, dcom, &b->com[3 * m], b->max_dcom[m])) continue;
+      double v, q;
+      if (molpair_energy(b, cbmc, si, x, y, z, m, &v, &q)) return 1;
+      if (combine) *evdw = *evdw + v + q;
+      else {
+        *eqq = *eqq + q;
+        *evdw = *evdw + v;
+      }
+    }
+  }
+  return 0;
+}
+
+/* Compute_Molecule_Nonbond_Intra_Energy (energy_routines.f90:980-1085) */
+static int mol_intra(const orc_box *b, int cbmc, int s, const double *x, const double *y,
+                     const double *z, double *ev, double *eq, double *eqi) {
+  const cass_topology *t = &b->t;
+  int o = t->species_atom_offset[s], n = t->species_natoms[s], po = t->species_pair_offset[s];
+  *ev = *eq = *eqi = 0.0;
+  for (int ia = 0; ia < n; ++ia) {
+    for (int ja = ia + 1; ja < n; ++ja) {
+      double rx = x[ia] - x[ja], ry = y[ia] - y[ja], rz = z[ia] - z[ja];
+      double r2 = rx * rx + ry * ry + rz * rz;
+      if (r2 <= b->p.rcut_lowsq && !t->bonded[po + ia * n + ja]) return 1;
+      int gv, gq;
+      atompair_cutoff(&b->p, cbmc, r2, &gv, &gq);
+      /* the reference drops q-q here when L_Coul_CBMC is TRUE (:1062) */
+      if (cbmc && t->species_coul_cbmc[s]) gq = 0;
+      if (gv || gq) {
+        double e[4];
+        atompair_energy(b, r2, t->atom_type[o + ia], t->atom_type[o + ja], t->atom_charge[o + ia],
+                        t->atom_charge[o + ja], 1, t->vdw_intra_scale[po + ia * n + ja],
+                        t->charge_intra_scale[po + ia * n + ja], gv, gq, e);
+        *ev = *ev + e[0];
+        *eq = *eq + e[1];
+        *eqi = *eqi + e[3];
+      }
+    }
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ Ewald */
+/* Ewald_Reciprocal_Lattice_Vector_Setup (energy_routines.f90:2170-2323) */
+static int ewald_setup(orc_box *b) {
+  const cass_params *p = &b->p;
+  double cv = 1.0 / (4.0 * p->alpha_ewald * p->alpha_ewald);
+  double hc2 = p->h_ewald_cut * p->h_ewald_cut;
+  int cap = 4096, n = 0;
+  double *buf = malloc(sizeof(double) * 5 * cap);
+  if (p->box_shape != CASS_BOX_CUBIC) {
+    const double *Li = p->length_inv;
+    for (int nz = -20; nz <= 20; ++nz)
+      for (int ny = -20; ny <= 20; ++ny)
+        for (int nx = 0; nx <= 20; ++nx) {
+          if (nx == 0 && ny == 0 && nz == 0) continue;
+          double X = nx, Y = ny, Z = nz;
+          double hxv = TWOPI_T * (Li[0] * X + Li[1] * Y + Li[2] * Z);
+          double hyv = TWOPI_T * (Li[3] * X + Li[4] * Y + Li[5] * Z);
+          double hzv = TWOPI_T * (Li[6] * X + Li[7] * Y + Li[8] * Z);
+          double h2 = hxv * hxv + hyv * hyv + hzv * hzv;
+          if (h2 < hc2) {
+            if (n == cap) { cap *= 2; buf = realloc(buf, sizeof(double) * 5 * cap); }
+            double c = (nx == 0 ? TWOPI_T : 2.0 * TWOPI_T) / p->volume * exp(-h2 * cv) / h2;
+            double *r = buf + 5 * n++;
+            r[0] = hxv; r[1] = hyv; r[2] = hzv; r[3] = h2; r[4] = c;
+          }
+        }
+  } else {
+    int kz = (int)(p->h_ewald_cut * p->basis_length[2] / TWOPI_T) + 1;
+    int ky = (int)(p->h_ewald_cut * p->basis_length[1] / TWOPI_T) + 1;
+    int kx = (int)(p->h_ewald_cut * p->basis_length[0] / TWOPI_T) + 1;
+    for (int nz = -kz; nz <= kz; ++nz)
+      for (int ny = -ky; ny <= ky; ++ny)
+        for (int nx = 0; nx <= kx; ++nx) {
+          if (n + 1 > MAXK) { free(buf); return CASS_ERR_KVEC; }
+          if (nx == 0 && ny == 0 && nz == 0) continue;
+          double hxv = TWOPI_T * (double)nx / p->basis_length[0];
+          double hyv = TWOPI_T * (double)ny / p->basis_length[1];
+          double hzv = TWOPI_T * (double)nz / p->basis_length[2];
+          double h2 = hxv * hxv + hyv * hyv + hzv * hzv;
+          if (h2 < hc2) {
+            if (n == cap) { cap *= 2; buf = realloc(buf, sizeof(double) * 5 * cap); }
+            double c = (nx == 0 ? TWOPI_T : 2.0 * TWOPI_T) / p->volume * exp(-h2 * cv) / h2;
+            double *r = buf + 5 * n++;
+            r[0] = hxv; r[1] = hyv; r[2] = hzv; r[3] = h2; r[4] = c;
+          }
+        }
+  }
+  b->nvecs = n;
+  double **arr[] = {&b->hx, &b->hy, &b->hz, &b->hsq, &b->cn, &b->cs, &b->sn, &b->cs_old, &b->sn_old};
+  for (int k = 0; k < 9; ++k) { free(*arr[k]); *arr[k] = calloc(n > 0 ? n : 1, sizeof(double)); }
+  for (int i = 0; i < n; ++i) {
+    b->hx[i] = buf[5 * i]; b->hy[i] = buf[5 * i + 1]; b->hz[i] = buf[5 * i + 2];
+    b->hsq[i] = buf[5 * i + 3]; b->cn[i] = buf[5 * i + 4];
+  }
+  free(buf);
+  free(b->cmol); free(b->smol); free(b->cmol_save); free(b->smol_save);
+  b->cmol = calloc((size_t)b->nmols * n + 1, sizeof(double));
+  b->smol = calloc((size_t)b->nmols * n + 1, sizeof(double));
+  b->cmol_save = calloc(n + 1, sizeof(double));
+  b->smol_save = calloc(n + 1, sizeof(double));
+  b->saved_mol = -1;
+  return 0;
+}
+
+static void mol_structure(const orc_box *b, int s, const double *x, const double *y, const double *z,
+                          int k, double *c, double *sn) {
+  int o = b->t.species_atom_offset[s], na = b->t.species_natoms[s];
+  double cc = 0.0, ss = 0.0;
+  for (int a = 0; a < na; ++a) {
+    double hr = b->hx[k] * x[a] + b->hy[k] * y[a] + b->hz[k] * z[a];
+    double q = b->t.atom_charge[o + a];
+    cc = cc + q * cos(hr);
+    ss = ss + q * sin(hr);
+  }
+  *c = cc;
+  *sn = ss;
+}
+
+/* Compute_System_Ewald_Reciprocal_Energy (energy_routines.f90:4111-4231) */
+static double ewald_full(orc_box *b) {
+  for (int i = 0; i < b->nvecs; ++i) b->cs[i] = b->sn[i] = 0.0;
+  for (int s = 0; s < b->t.nspecies; ++s) {
+    if (!species_has_charge(&b->t, s)) continue;
+    for (int m = 0; m < b->nmols; ++m) {
+      if (b->mol_species[m] != s || !b->mol_live[m]) continue;
+      int f = b->mol_first[m];
+      for (int i = 0; i < b->nvecs; ++i) {
+        double c, sn;
+        mol_structure(b, s, b->x + f, b->y + f, b->z + f, i, &c, &sn);
+        b->cmol[(size_t)m * b->nvecs + i] = c;
+        b->smol[(size_t)m * b->nvecs + i] = sn;
+        b->cs[i] = b->cs[i] + c;
+        b->sn[i] = b->sn[i] + sn;
+      }
+    }
+  }
+  double E = 0.0;
+  for (int i = 0; i < b->nvecs; ++i) E = E + b->cn[i] * (b->cs[i] * b->cs[i] + b->sn[i] * b->sn[i]);
+  return E * CHARGE_FACTOR;
+}
+
+/* Update_System_Ewald_Reciprocal_Energy (energy_routines.f90:2326-2497) */
+static double ewald_move(orc_box *b, int m, int flag, const double *x, const double *y, const double *z) {
+  int K = b->nvecs, s = b->mol_species[m];
+  memcpy(b->cs_old, b->cs, sizeof(double) * K);
+  memcpy(b->sn_old, b->sn, sizeof(double) * K);
+  double *cm = b->cmol + (size_t)m * K, *sm = b->smol + (size_t)m * K;
+  memcpy(b->cmol_save, cm, sizeof(double) * K);
+  memcpy(b->smol_save, sm, sizeof(double) * K);
+  b->saved_mol = m;
+  double E = 0.0;
+  if (flag == CASS_MOVE_DELETION) {
+    for (int i = 0; i < K; ++i) {
+      b->cs[i] = b->cs[i] - cm[i];
+      b->sn[i] = b->sn[i] - sm[i];
+    }
+    for (int i = 0; i < K; ++i) E = E + b->cn[i] * (b->cs[i] * b->cs[i] + b->sn[i] * b->sn[i]);
+  } else if (flag == CASS_MOVE_INSERTION) {
+    for (int i = 0; i < K; ++i) {
+      mol_structure(b, s, x, y, z, i, &cm[i], &sm[i]);
+      b->cs[i] = b->cs[i] + cm[i];
+      b->sn[i] = b->sn[i] + sm[i];
+      E = E + b->cn[i] * (b->cs[i] * b->cs[i] + b->sn[i] * b->sn[i]);
+    }
+  } else {
+    for (int i = 0; i < K; ++i) {
+      double c, sn;
+      mol_structure(b, s, x, y, z, i, &c, &sn);
+      b->cs[i] = b->cs[i] + (c - cm[i]);
+      b->sn[i] = b->sn[i] + (sn - sm[i]);
+      E = E + b->cn[i] * (b->cs[i] * b->cs[i] + b->sn[i] * b->sn[i]);
+      cm[i] = c;
+      sm[i] = sn;
+    }
+  }
+  return E * CHARGE_FACTOR;
+}
+
+static void ewald_revert(orc_box *b) {
+  /* caller restore, move_translate.f90:296-306 */
+  int K = b->nvecs;
+  if (b->saved_mol < 0) return;
+  memcpy(b->cs, b->cs_old, sizeof(double) * K);
+  memcpy(b->sn, b->sn_old, sizeof(double) * K);
+  memcpy(b->cmol + (size_t)b->saved_mol * K, b->cmol_save, sizeof(double) * K);
+  memcpy(b->smol + (size_t)b->saved_mol * K, b->smol_save, sizeof(double) * K);
+  b->saved_mol = -1;
+}
+
+/* Update_System_Ewald_Reciprocal_Energy_Widom (energy_routines.f90:2500-2550) */
+static double ewald_widom(const orc_box *b, int s, const double *x, const double *y, const double *z) {
+  double E = 0.0;
+  int o = b->t.species_atom_offset[s], n = b->t.species_natoms[s];
+  for (int i = 0; i < b->nvecs; ++i) {
+    double dc = 0.0, ds = 0.0;
+    for (int a = 0; a < n; ++a) {
+      double hr = b->hx[i] * x[a] + b->hy[i] * y[a] + b->hz[i] * z[a];
+      dc = dc + b->t.atom_charge[o + a] * cos(hr);
+    }
+    for (int a = 0; a < n; ++a) {
+      double hr = b->hx[i] * x[a] + b->hy[i] * y[a] + b->hz[i] * z[a];
+      ds = ds + b->t.atom_charge[o + a] * sin(hr);
+    }
+    double c = b->cs[i] + dc, sn = b->sn[i] + ds;
+    E = E + b->cn[i] * (c * c + sn * sn);
+  }
+  return E * CHARGE_FACTOR;
+}
+
+/* Compute_LR_Correction (energy_routines.f90:2956-3023) */
+static double lrc(const orc_box *b, const double *nbeads) {
+  const cass_topology *t = &b->t;
+  const cass_params *p = &b->p;
+  int nt = t->ntypes;
+  double e = 0.0;
+  if (p->vdw_style == CASS_VDW_LJ) {
+    for (int i = 0; i < nt; ++i) {
+      double eij = 0.0;
+      for (int j = 0; j < nt; ++j) {
+        double eps = t->eps[i * nt + j], sig = t->sig[i * nt + j];
+        double s2 = sig * sig, s6 = s2 * s2 * s2, s12 = s6 * s6;
+        eij = eij + nbeads[j] * 4.0 * eps * (s12 / (9.0 * p->rcut9) - (s6 / (3.0 * p->rcut3)));
+      }
+      e = e + nbeads[i] * eij;
+    }
+    e = 2.0 * PI_T * e / p->volume;
+  } else if (p->vdw_style == CASS_VDW_MIE) {
+    for (int i = 0; i < nt; ++i) {
+      double eij = 0.0;
+      for (int j = 0; j < nt; ++j) {
+        double eps = t->eps[i * nt + j], sig = t->sig[i * nt + j];
+        double n = t->mie_n[i * nt + j], m = t->mie_m[i * nt + j];
+        double coeff = n / (n - m) * pow(n / m, m / (n - m));
+        double sr = sig / p->rcut_vdw;
+        double srn = pow(sr, n), srm = pow(sr, m);
+        eij = eij + nbeads[j] * coeff * eps * pow(p->rcut_vdw, 3.0) * ((srn / (3.0 - n)) + (srm / (m - 3.0)));
+      }
+      e = e + nbeads[i] * eij;
+    }
+    e = -2.0 * PI_T * e / p->volume;
+  }
+  return e;
+}
+
+static double species_self(const orc_box *b, int s) {
+  /* Compute_Molecule_Self_Energy (energy_routines.f90:2598-2645) */
+  int o = b->t.species_atom_offset[s], n = b->t.species_natoms[s];
+  double qq = 0.0;
+  for (int a = 0; a < n; ++a) qq = qq + b->t.atom_charge[o + a] * b->t.atom_charge[o + a];
+  if (b->p.charge_style == CASS_Q_EWALD) return -qq * CHARGE_FACTOR * b->p.alpha_ewald / ROOTPI_T;
+  if (b->p.charge_style == CASS_Q_DSF)
+    return -qq * (b->p.alpha_dsf / ROOTPI_T + b->p.dsf_factor1 / 2.0) * CHARGE_FACTOR;
+  return 0.0;
+}
+
+static void count_beads(const orc_box *b, double *nb) {
+  for (int i = 0; i < b->t.ntypes; ++i) nb[i] = 0.0;
+  for (int m = 0; m < b->nmols; ++m) {
+    if (!b->mol_live[m]) continue;
+    int s = b->mol_species[m], o = b->t.species_atom_offset[s];
+    for (int a = 0; a < b->t.species_natoms[s]; ++a) {
+      int ty = b->t.atom_type[o + a];
+      if (ty >= 0) nb[ty] += 1.0;
+    }
+  }
+}
+
+/* ============================================================ public API */
+orc_box *orc_box_create(const cass_params *p, const cass_topology *t, int nmols, const int *species,
+                        const int *live, const double *xyz) {
+  orc_box *b = calloc(1, sizeof(orc_box));
+  b->p = *p;
+  b->t = *t;
+  b->nmols = nmols;
+  b->mol_species = malloc(sizeof(int) * (nmols + 1));
+  b->mol_live = malloc(sizeof(int) * (nmols + 1));
+  b->mol_first = malloc(sizeof(int) * (nmols + 1));
+  int ns = 0;
+  for (int m = 0; m < nmols; ++m) {
+    b->mol_species[m] = species[m];
+    b->mol_live[m] = live[m];
+    b->mol_first[m] = ns;
+    ns += t->species_natoms[species[m]];
+  }
+  b->nsites = ns;
+  b->x = malloc(sizeof(double) * (ns + 1));
+  b->y = malloc(sizeof(double) * (ns + 1));
+  b->z = malloc(sizeof(double) * (ns + 1));
+  for (int i = 0; i < ns; ++i) {
+    b->x[i] = xyz[3 * i];
+    b->y[i] = xyz[3 * i + 1];
+    b->z[i] = xyz[3 * i + 2];
+  }
+  b->com = malloc(sizeof(double) * 3 * (nmols + 1));
+  b->max_dcom = malloc(sizeof(double) * (nmols + 1));
+  for (int m = 0; m < nmols; ++m) {
+    int f = b->mol_first[m];
+    get_com(b, b->mol_species[m], b->x + f, b->y + f, b->z + f, &b->com[3 * m], &b->max_dcom[m]);
+  }
+  b->saved_mol = -1;
+  return b;
+}
+
+void orc_box_destroy(orc_box *b) {
+  if (!b) return;
+  void *ps[] = {b->mol_species, b->mol_live, b->mol_first, b->x, b->y, b->z, b->com, b->max_dcom,
+                b->hx, b->hy, b->hz, b->hsq, b->cn, b->cs, b->sn, b->cs_old, b->sn_old, b->cmol,
+                b->smol, b->cmol_save, b->smol_save};
+  for (unsigned k = 0; k < sizeof(ps) / sizeof(ps[0]); ++k) free(ps[k]);
+  free(b);
+}
+
+/* Fold_Molecule for all molecules (main.f90:367-376 start-up loop) */
+void orc_fold_all(orc_box *b) {
+  for (int m = 0; m < b->nmols; ++m) {
+    if (b->p.box_shape != CASS_BOX_CUBIC) continue;
+    int s = b->mol_species[m], f = b->mol_first[m], n = b->t.species_natoms[s];
+    for (int d = 0; d < 3; ++d) {
+      double L = b->p.cell[4 * d], h = b->p.hlength[d];
+      double *c = &b->com[3 * m + d];
+      double *r = d == 0 ? b->x : (d == 1 ? b->y : b->z);
+      if (*c > h) { *c = *c - L; for (int a = 0; a < n; ++a) r[f + a] = r[f + a] - L; }
+      else if (*c < -h) { *c = *c + L; for (int a = 0; a < n; ++a) r[f + a] = r[f + a] + L; }
+    }
+  }
+}
+
+void orc_get_xyz(const orc_box *b, double *xyz) {
+  for (int i = 0; i < b->nsites; ++i) {
+    xyz[3 * i] = b->x[i];
+    xyz[3 * i + 1] = b->y[i];
+    xyz[3 * i + 2] = b->z[i];
+  }
+}
+
+void orc_get_com(const orc_box *b, double *com, double *dcom) {
+  memcpy(com, b->com, sizeof(double) * 3 * b->nmols);
+  memcpy(dcom, b->max_dcom, sizeof(double) * b->nmols);
+}
+
+int orc_mol_inter_energy(const orc_box *b, int m, int cbmc, double *evdw, double *eqq) {
+  int f = b->mol_first[m];
+  return mol_inter(b, cbmc, b->mol_species[m], b->x + f, b->y + f, b->z + f, &b->com[3 * m],
+                   b->max_dcom[m], m, 0, evdw, eqq);
+}
+
+int orc_mol_intra_energy(const orc_box *b, int m, int cbmc, double *ev, double *eq, double *eqi) {
+  int f = b->mol_first[m];
+  return mol_intra(b, cbmc, b->mol_species[m], b->x + f, b->y + f, b->z + f, ev, eq, eqi);
+}
+
+int orc_ewald_setup(orc_box *b) { return ewald_setup(b); }
+double orc_ewald_full(orc_box *b) { return ewald_full(b); }
+int orc_ewald_nvecs(const orc_box *b) { return b->nvecs; }
+void orc_ewald_tables(const orc_box *b, double *hx, double *hy, double *hz, double *cn, double *cs,
+                      double *sn) {
+  for (int i = 0; i < b->nvecs; ++i) {
+    hx[i] = b->hx[i]; hy[i] = b->hy[i]; hz[i] = b->hz[i];
+    cn[i] = b->cn[i]; cs[i] = b->cs[i]; sn[i] = b->sn[i];
+  }
+}
+
+double orc_lrc(const orc_box *b, const double *nbeads) { return lrc(b, nbeads); }
+double orc_species_self(const orc_box *b, int s) { return species_self(b, s); }
+
+/* Compute_System_Total_Energy (energy_routines.f90:2648-2953), nonbonded part;
+ * e[] in cass_energy field order. */
+int orc_total_energy(orc_box *b, cass_energy *e) {
+  memset(e, 0, sizeof(*e));
+  for (int s = 0; s < b->t.nspecies; ++s) {
+    double vi = 0, vq = 0, vqi = 0;
+    for (int m = 0; m < b->nmols; ++m) {
+      if (b->mol_species[m] != s || !b->mol_live[m]) continue;
+      double a, q, qi;
+      if (orc_mol_intra_energy(b, m, 0, &a, &q, &qi)) return 1;
+      vi = vi + q + a;
+      vq = vq + q;
+      vqi = vqi + qi;
+      e->intra_vdw = e->intra_vdw + a;
+    }
+    e->intra = e->intra + vi;
+    e->intra_q = e->intra_q + vq;
+    e->inter_q = e->inter_q + vqi;
+  }
+  /* same-species pairs i<j, then cross-species pairs */
+  for (int s = 0; s < b->t.nspecies; ++s)
+    for (int m1 = 0; m1 < b->nmols; ++m1) {
+      if (b->mol_species[m1] != s || !b->mol_live[m1]) continue;
+      double ev = 0, eq = 0;
+      for (int m2 = m1 + 1; m2 < b->nmols; ++m2) {
+        if (b->mol_species[m2] != s || !b->mol_live[m2]) continue;
+        if (!molpair_cutoff(b, 0, &b->com[3 * m1], b->max_dcom[m1], &b->com[3 * m2], b->max_dcom[m2]))
+          continue;
+        double v, q;
+        int f = b->mol_first[m1];
+        if (molpair_energy(b, 0, s, b->x + f, b->y + f, b->z + f, m2, &v, &q)) return 1;
+        ev = ev + v;
+        eq = eq + q;
+      }
+      e->inter_vdw = e->inter_vdw + ev;
+      e->inter_q = e->inter_q + eq;
+    }
+  for (int s1 = 0; s1 < b->t.nspecies; ++s1)
+    for (int m1 = 0; m1 < b->nmols; ++m1) {
+      if (b->mol_species[m1] != s1 || !b->mol_live[m1]) continue;
+      for (int s2 = s1 + 1; s2 < b->t.nspecies; ++s2) {
+        double ev = 0, eq = 0;
+        for (int m2 = 0; m2 < b->nmols; ++m2) {
+          if (b->mol_species[m2] != s2 || !b->mol_live[m2]) continue;
+          if (!molpair_cutoff(b, 0, &b->com[3 * m1], b->max_dcom[m1], &b->com[3 * m2], b->max_dcom[m2]))
+            continue;
+          double v, q;
+          int f = b->mol_first[m1];
+          if (molpair_energy(b, 0, s1, b->x + f, b->y + f, b->z + f, m2, &v, &q)) return 1;
+          ev = ev + v;
+          eq = eq + q;
+        }
+        e->inter_vdw = e->inter_vdw + ev;
+        e->inter_q = e->inter_q + eq;
+      }
+    }
+  e->inter = e->inter_q + e->inter_vdw;
+  if (b->p.charge_style != CASS_Q_NONE) {
+    if (b->p.charge_style == CASS_Q_EWALD) {
+      if (b->nvecs == 0 && ewald_setup(b)) return 2;
+      e->reciprocal = ewald_full(b);
+      e->inter = e->inter + e->reciprocal;
+    }
+    double qq = 0.0;
+    for (int s = 0; s < b->t.nspecies; ++s)
+      for (int m = 0; m < b->nmols; ++m) {
+        if (b->mol_species[m] != s || !b->mol_live[m]) continue;
+        int o = b->t.species_atom_offset[s];
+        for (int a = 0; a < b->t.species_natoms[s]; ++a) qq = qq + b->t.atom_charge[o + a] * b->t.atom_charge[o + a];
+      }
+    if (b->p.charge_style == CASS_Q_EWALD) e->self = -qq * CHARGE_FACTOR * b->p.alpha_ewald / ROOTPI_T;
+    else if (b->p.charge_style == CASS_Q_DSF)
+      e->self = -(qq * (b->p.alpha_dsf / ROOTPI_T + b->p.dsf_factor1 / 2.0)) * CHARGE_FACTOR;
+    e->inter = e->inter + e->self;
+  }
+  if (b->p.vdw_sum_style == CASS_SUM_CUT_TAIL) {
+    double *nb = calloc(b->t.ntypes + 1, sizeof(double));
+    count_beads(b, nb);
+    e->lrc = lrc(b, nb);
+    free(nb);
+    e->inter = e->inter + e->lrc;
+  }
+  e->total = e->intra + e->inter;
+  return 0;
+}
+
+/* Widom energies of explicit placements (the per-insertion work of
+ * widom_insert.f90:378-431 without CBMC placement).  out per placement:
+ * {E_inter(full, combined), E_periodic_qq, E_intra_vdw, E_intra_qq, E_recip,
+ *  E_inter(cbmc)}; ov per placement: {overlap_full, overlap_cbmc}. */
+void orc_widom_energies(orc_box *b, int s, int n, const double *xyz, double *out, int *ov) {
+  int na = b->t.species_natoms[s];
+  double x[64], y[64], z[64], com[3], dcom;
+  for (int i = 0; i < n; ++i) {
+    for (int a = 0; a < na; ++a) {
+      x[a] = xyz[(size_t)3 * (i * na + a)];
+      y[a] = xyz[(size_t)3 * (i * na + a) + 1];
+      z[a] = xyz[(size_t)3 * (i * na + a) + 2];
+    }
+    get_com(b, s, x, y, z, com, &dcom);
+    double e, dummy, *o = out + 6 * (size_t)i;
+    int over = mol_inter(b, 0, s, x, y, z, com, dcom, -1, 1, &e, &dummy);
+    o[0] = e;
+    o[1] = o[2] = o[3] = 0.0;
+    if (!over) {
+      double ev, eq, eqi;
+      mol_intra(b, 0, s, x, y, z, &ev, &eq, &eqi);
+      o[1] = eqi; o[2] = ev; o[3] = eq;
+    }
+    o[4] = (b->p.charge_style == CASS_Q_EWALD && species_has_charge(&b->t, s)) ? ewald_widom(b, s, x, y, z) : 0.0;
+    int over_c = mol_inter(b, 1, s, x, y, z, com, dcom, -1, 1, &e, &dummy);
+    o[5] = e;
+    ov[2 * i] = over;
+    ov[2 * i + 1] = over_c;
+  }
+}
+
+/* ------------------------------------------------------------------- RNG */
+/* random_generators.f90:51-122 (L'Ecuyer 1999 five-component LFSR).
+ * Fortran ISHFT with a negative count is a logical right shift. */
+static inline uint64_t lsr(int64_t v, int k) { return (uint64_t)v >> k; }
+static inline int64_t lsl(int64_t v, int k) { return (int64_t)((uint64_t)v << k); }
+
+void orc_rng_init(int64_t s[5], int64_t i1, int64_t i3) {
+  s[0] = i1; s[1] = -759022222; s[2] = i3; s[3] = -1718083407; s[4] = -123456789;
+  if ((s[0] & -2) == 0) s[0] = i1 - 8388607;
+  if ((s[2] & -4096) == 0) s[2] = i3 - 8388607;
+}
+
+int64_t orc_rng_int(int64_t s[5]) {
+  int64_t b;
+  b = (int64_t)lsr(lsl(s[0], 1) ^ s[0], 53);
+  s[0] = lsl(s[0] & -2, 10) ^ b;
+  b = (int64_t)lsr(lsl(s[1], 24) ^ s[1], 50);
+  s[1] = lsl(s[1] & -512, 5) ^ b;
+  b = (int64_t)lsr(lsl(s[2], 3) ^ s[2], 23);
+  s[2] = lsl(s[2] & -4096, 29) ^ b;
+  b = (int64_t)lsr(lsl(s[3], 5) ^ s[3], 24);
+  s[3] = lsl(s[3] & -131072, 23) ^ b;
+  b = (int64_t)lsr(lsl(s[4], 3) ^ s[4], 33);
+  s[4] = lsl(s[4] & -8388608, 8) ^ b;
+  return s[0] ^ s[1] ^ s[2] ^ s[3] ^ s[4];
+}
+
+double orc_rng_uniform(int64_t s[5]) {
+  return (double)orc_rng_int(s) * 5.4210108624275221E-20 + 0.5;
+}
+
+/* accept_or_reject.f90:22-65 */
+int orc_accept(int64_t s[5], double ln_pacc) {
+  if (ln_pacc <= 0.0) return 1;
+  if (ln_pacc < MAX_KBT) {
+    double pacc = exp(-ln_pacc);
+    if (orc_rng_uniform(s) <= pacc) return 1;
+  }
+  return 0;
+}
+
+/* -------------------------------------------------------------- NVT chain */
+/* nvtmc_driver.f90:94-152 with Translate (move_translate.f90:23-360) and
+ * Rotate (move_rotate.f90:23-520) for a one-box, one-species system.
+ * e[] holds the running Energy_Class; kind/acc/etot per step (may be NULL). */
+int orc_nvt_run(orc_box *b, int64_t s[5], double prob_trans, double max_disp, double max_rot,
+                int linear, int64_t nsteps, cass_energy *e, uint8_t *kind, uint8_t *acc,
+                double *etot) {
+  const int sp = 0;
+  int na = b->t.species_natoms[sp];
+  int charged = b->p.charge_style == CASS_Q_EWALD && species_has_charge(&b->t, sp);
+  double xo[64], yo[64], zo[64], co[3], dco;
+  int nm = 0;
+  for (int m = 0; m < b->nmols; ++m) nm += (b->mol_species[m] == sp && b->mol_live[m]);
+  for (int64_t step = 0; step < nsteps; ++step) {
+    double rno = orc_rng_uniform(s);
+    int rot = !(rno <= prob_trans);
+    if (kind) kind[step] = (uint8_t)rot;
+    (void)orc_rng_uniform(s);                   /* species pick (:158 / rotate :103) */
+    int im = (int)(orc_rng_uniform(s) * nm);  /* molecule pick */
+    int m = im;                                 /* locate = identity for a packed box */
+    int f = b->mol_first[m];
+    double Ev, Eq, Evn, Eqn;
+    if (orc_mol_inter_energy(b, m, 0, &Ev, &Eq)) return -1; /* reference aborts */
+    for (int a = 0; a < na; ++a) { xo[a] = b->x[f + a]; yo[a] = b->y[f + a]; zo[a] = b->z[f + a]; }
+    co[0] = b->com[3 * m]; co[1] = b->com[3 * m + 1]; co[2] = b->com[3 * m + 2]; dco = b->max_dcom[m];
+    if (!rot) {
+      double dx = (2.0 * orc_rng_uniform(s) - 1.0) * max_disp;
+      double dy = (2.0 * orc_rng_uniform(s) - 1.0) * max_disp;
+      double dz = (2.0 * orc_rng_uniform(s) - 1.0) * max_disp;
+      for (int a = 0; a < na; ++a) { b->x[f + a] += dx; b->y[f + a] += dy; b->z[f + a] += dz; }
+      b->com[3 * m] += dx; b->com[3 * m + 1] += dy; b->com[3 * m + 2] += dz;
+      /* Fold_Molecule (minimum_image_separation.f90:162-240), cubic */
+      for (int d = 0; d < 3; ++d) {
+        double L = b->p.cell[4 * d], h = b->p.hlength[d];
+        double *c = &b->com[3 * m + d];
+        double *r = d == 0 ? b->x : (d == 1 ? b->y : b->z);
+        if (*c > h) { *c = *c - L; for (int a = 0; a < na; ++a) r[f + a] = r[f + a] - L; }
+        else if (*c < -h) { *c = *c + L; for (int a = 0; a < na; ++a) r[f + a] = r[f + a] + L; }
+      }
+    } else {
+      double r11, r12, r13, r21, r22, r23, r31, r32, r33;
+      if (linear) {
+        /* Rotate_Molecule_Eulerian (move_rotate.f90:468-520) */
+        double theta = acos(1.0 - 2.0 * orc_rng_uniform(s));
+        double phi = (1.0 - 2.0 * orc_rng_uniform(s)) * PI_T;
+        double psi = (1.0 - 2.0 * orc_rng_uniform(s)) * PI_T;
+        r11 = cos(psi) * cos(phi) - cos(theta) * sin(phi) * sin(psi);
+        r12 = cos(psi) * sin(phi) + cos(theta) * cos(phi) * sin(psi);
+        r13 = sin(psi) * sin(theta);
+        r21 = -sin(psi) * cos(phi) - cos(theta) * sin(phi) * cos(psi);
+        r22 = -sin(psi) * sin(phi) + cos(theta) * cos(phi) * cos(psi);
+        r23 = cos(psi) * sin(theta);
+        r31 = sin(theta) * sin(phi);
+        r32 = -sin(theta) * cos(phi);
+        r33 = cos(theta);
+      } else {
+        /* Rotate_Molecule_Axis (move_rotate.f90:362-463) */
+        double p1 = 0, p2 = 0, p3 = 0;
+        int axis = (int)(3.0 * orc_rng_uniform(s)) + 1;
+        double ang = (2.0 * orc_rng_uniform(s) - 1.0) * max_rot;
+        if (axis == 1) p1 = ang; else if (axis == 2) p2 = ang; else p3 = ang;
+        double c1 = cos(p1), c2 = cos(p2), c3 = cos(p3), s1 = sin(p1), s2 = sin(p2), s3 = sin(p3);
+        r11 = c1 * c3 - s1 * c2 * s3;
+        r12 = s1 * c3 + c1 * c2 * s3;
+        r13 = s2 * s3;
+        r21 = -c1 * s3 - s1 * c2 * c3;
+        r22 = -s1 * s3 + c1 * c2 * c3;
+        r23 = s2 * c3;
+        r31 = s1 * s2;
+        r32 = -c1 * s2;
+        r33 = c2;
+      }
+      for (int a = 0; a < na; ++a) {
+        double px = b->x[f + a] - b->com[3 * m], py = b->y[f + a] - b->com[3 * m + 1],
+               pz = b->z[f + a] - b->com[3 * m + 2];
+        double nx = r11 * px + r12 * py + r13 * pz;
+        double ny = r21 * px + r22 * py + r23 * pz;
+        double nz = r31 * px + r32 * py + r33 * pz;
+        b->x[f + a] = nx + b->com[3 * m];
+        b->y[f + a] = ny + b->com[3 * m + 1];
+        b->z[f + a] = nz + b->com[3 * m + 2];
+      }
+    }
+    int over = orc_mol_inter_energy(b, m, 0, &Evn, &Eqn);
+    int accept = 0;
+    if (!over) {
+      double dE = 0.0, Erm = 0.0;
+      if (charged) {
+        Erm = ewald_move(b, m, rot ? CASS_MOVE_ROTATION : CASS_MOVE_TRANSLATION, b->x + f, b->y + f, b->z + f);
+        dE = Erm - e->reciprocal;
+      }
+      if (!rot) dE = dE + (Evn - Ev) + (Eqn - Eq);
+      else dE = dE + Evn - Ev + Eqn - Eq;
+      double ln_pacc = b->p.beta * dE;
+      accept = orc_accept(s, ln_pacc);
+      if (accept) {
+        e->total = e->total + dE;
+        e->inter = e->inter + dE;
+        e->inter_vdw = e->inter_vdw + Evn - Ev;
+        e->inter_q = e->inter_q + Eqn - Eq;
+        if (charged) e->reciprocal = Erm;
+        if (rot) get_com(b, sp, b->x + f, b->y + f, b->z + f, &b->com[3 * m], NULL);
+        b->saved_mol = -1;
+      } else if (charged) {
+        ewald_revert(b);
+      }
+    }
+    if (!accept) {
+      for (int a = 0; a < na; ++a) { b->x[f + a] = xo[a]; b->y[f + a] = yo[a]; b->z[f + a] = zo[a]; }
+      b->com[3 * m] = co[0]; b->com[3 * m + 1] = co[1]; b->com[3 * m + 2] = co[2]; b->max_dcom[m] = dco;
+    }
+    if (acc) acc[step] = (uint8_t)accept;
+    if (etot) etot[step] = e->total;
+  }
+  return 0;
+}

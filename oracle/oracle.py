@@ -1,0 +1,178 @@
+"""ctypes front end of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from cassandra_amd import abi
+from cassandra_amd import systems as S
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _lib():
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    d, i32, i64 = C.c_double, C.c_int32, C.c_int64
+    lib.orc_params_derive.argtypes = [P(abi.CassParams)]
+    lib.orc_box_create.restype = C.c_void_p
+    lib.orc_box_create.argtypes = [P(abi.CassParams), P(abi.CassTopology), C.c_int, P(i32), P(i32), P(d)]
+    lib.orc_box_destroy.argtypes = [C.c_void_p]
+    lib.orc_fold_all.argtypes = [C.c_void_p]
+    lib.orc_get_xyz.argtypes = [C.c_void_p, P(d)]
+    lib.orc_get_com.argtypes = [C.c_void_p, P(d), P(d)]
+    lib.orc_mol_inter_energy.argtypes = [C.c_void_p, C.c_int, C.c_int, P(d), P(d)]
+    lib.orc_mol_intra_energy.argtypes = [C.c_void_p, C.c_int, C.c_int, P(d), P(d), P(d)]
+    lib.orc_ewald_setup.argtypes = [C.c_void_p]
+    lib.orc_ewald_full.restype = d
+    lib.orc_ewald_full.argtypes = [C.c_void_p]
+    lib.orc_ewald_nvecs.argtypes = [C.c_void_p]
+    lib.orc_ewald_tables.argtypes = [C.c_void_p] + [P(d)] * 6
+    lib.orc_lrc.restype = d
+    lib.orc_lrc.argtypes = [C.c_void_p, P(d)]
+    lib.orc_species_self.restype = d
+    lib.orc_species_self.argtypes = [C.c_void_p, C.c_int]
+    lib.orc_total_energy.argtypes = [C.c_void_p, P(abi.CassEnergy)]
+    lib.orc_widom_energies.argtypes = [C.c_void_p, C.c_int, C.c_int, P(d), P(d), P(C.c_int)]
+    lib.orc_rng_init.argtypes = [P(i64), i64, i64]
+    lib.orc_rng_int.restype = i64
+    lib.orc_rng_int.argtypes = [P(i64)]
+    lib.orc_rng_uniform.restype = d
+    lib.orc_rng_uniform.argtypes = [P(i64)]
+    lib.orc_accept.argtypes = [P(i64), d]
+    lib.orc_nvt_run.argtypes = [C.c_void_p, P(i64), d, d, d, C.c_int, i64, P(abi.CassEnergy),
+                                P(C.c_uint8), P(C.c_uint8), P(d)]
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _lib()
+    return _LIB
+
+
+def derive(system: S.System) -> abi.CassParams:
+    p = abi.raw_params(system)
+    lib().orc_params_derive(C.byref(p))
+    return p
+
+
+class OracleBox:
+    """One box in the oracle, built from a System."""
+
+    def __init__(self, system: S.System, fold=True):
+        self.system = system
+        self.topo = abi.Topology(system)
+        self.params = derive(system)
+        spec, live, xyz = abi.box_arrays(system)
+        self.nmols = len(spec)
+        self.spec, self.live = spec, live
+        self.h = lib().orc_box_create(C.byref(self.params), C.byref(self.topo.c), self.nmols,
+                                      abi.ptr(spec, C.c_int32), abi.ptr(live, C.c_int32), abi.ptr(xyz))
+        self.nsites = xyz.size // 3
+        if fold:
+            lib().orc_fold_all(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_box_destroy(self.h)
+            self.h = None
+
+    def xyz(self):
+        out = np.zeros(3 * self.nsites)
+        lib().orc_get_xyz(self.h, abi.ptr(out))
+        return out
+
+    def com(self):
+        c = np.zeros(3 * self.nmols)
+        d = np.zeros(self.nmols)
+        lib().orc_get_com(self.h, abi.ptr(c), abi.ptr(d))
+        return c, d
+
+    def mol_inter(self, m, cbmc=False):
+        ev, eq = C.c_double(), C.c_double()
+        ov = lib().orc_mol_inter_energy(self.h, m, int(cbmc), C.byref(ev), C.byref(eq))
+        return ev.value, eq.value, bool(ov)
+
+    def mol_intra(self, m, cbmc=False):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        ov = lib().orc_mol_intra_energy(self.h, m, int(cbmc), C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value, bool(ov)
+
+    def ewald_setup(self):
+        rc = lib().orc_ewald_setup(self.h)
+        if rc:
+            raise RuntimeError("k-vector setup failed")
+        return lib().orc_ewald_nvecs(self.h)
+
+    def ewald_tables(self):
+        n = lib().orc_ewald_nvecs(self.h)
+        arrs = [np.zeros(n) for _ in range(6)]
+        lib().orc_ewald_tables(self.h, *[abi.ptr(a) for a in arrs])
+        return dict(zip(["hx", "hy", "hz", "cn", "cos_sum", "sin_sum"], arrs))
+
+    def total_energy(self):
+        e = abi.CassEnergy()
+        rc = lib().orc_total_energy(self.h, C.byref(e))
+        if rc:
+            raise RuntimeError("overlap in configuration")
+        return e.as_dict()
+
+    def widom_energies(self, species, placements):
+        placements = np.ascontiguousarray(placements, dtype=np.float64)
+        n = placements.shape[0]
+        out = np.zeros((n, 6))
+        ov = np.zeros((n, 2), dtype=np.int32)
+        lib().orc_widom_energies(self.h, species, n, abi.ptr(placements), abi.ptr(out), abi.ptr(ov, C.c_int))
+        return out, ov.astype(bool)
+
+    def lrc(self, nbeads):
+        nb = np.ascontiguousarray(nbeads, dtype=np.float64)
+        return lib().orc_lrc(self.h, abi.ptr(nb))
+
+    def species_self(self, s):
+        return lib().orc_species_self(self.h, s)
+
+    def nvt_run(self, rng_state, nsteps, energy: dict, prob_translate, max_disp, max_rot, linear):
+        e = abi.CassEnergy(**{k: energy.get(k, 0.0) for k in abi.ENERGY_FIELDS})
+        kind = np.zeros(nsteps, dtype=np.uint8)
+        acc = np.zeros(nsteps, dtype=np.uint8)
+        et = np.zeros(nsteps)
+        rc = lib().orc_nvt_run(self.h, abi.ptr(rng_state, C.c_int64), prob_translate, max_disp, max_rot,
+                               int(linear), nsteps, C.byref(e), abi.ptr(kind, C.c_uint8),
+                               abi.ptr(acc, C.c_uint8), abi.ptr(et))
+        if rc:
+            raise RuntimeError("overlap in the current configuration")
+        return kind, acc.astype(bool), et, e.as_dict()
+
+
+def rng_init(i1, i3):
+    s = np.zeros(5, dtype=np.int64)
+    lib().orc_rng_init(abi.ptr(s, C.c_int64), i1, i3)
+    return s
+
+
+def rng_uniform(s, n):
+    return np.array([lib().orc_rng_uniform(abi.ptr(s, C.c_int64)) for _ in range(n)])
+
+
+def rng_int(s, n):
+    return [lib().orc_rng_int(abi.ptr(s, C.c_int64)) for _ in range(n)]

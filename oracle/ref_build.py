@@ -1,0 +1,109 @@
+#!/usr/bin/env python3
+"""Build the Cassandra Fortran reference from its own sources into oracle/_ref/.
+
+Test infrastructure only.  The reference sources are compiled where they lie
+(``/root/reference/Src``); nothing is copied.  Objects, module files, the
+reference executable (``cassandra_ref.exe``) and the golden-vector probe
+(``ref_probe``, built from ``oracle/ref_probe.f90``, which only calls the
+reference's own subroutines) all go to ``oracle/_ref/``.
+
+The reference's own build system (Makefile + get_deps) is not used: module
+dependencies are found here by scanning MODULE / USE statements, and the XTC
+reader is replaced by the reference's own ``xtc_routines.no_xtc.f90`` so no
+external library is needed.
+
+Usage: python3 oracle/ref_build.py [--src /root/reference/Src] [-j 8]
+"""
+import argparse
+import os
+import re
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "_ref")
+FC = "/opt/rocm/lib/llvm/bin/amdflang"
+# Optimised OpenMP build.  No -march: baseline x86-64 has no FMA instruction,
+# so every multiply-add rounds twice, as the source is written.
+FFLAGS = ["-O2", "-fopenmp"]
+
+SKIP = {"min.f90", "xtc_routines.f90"}  # dead duplicate / needs libgmxfort
+
+
+def scan(path):
+    mods, uses = set(), set()
+    with open(path, errors="replace") as f:
+        for line in f:
+            s = line.split("!")[0].strip().lower()
+            m = re.match(r"module\s+(\w+)\s*$", s)
+            if m and m.group(1) != "procedure":
+                mods.add(m.group(1))
+            m = re.match(r"use\s*(?:,\s*intrinsic\s*::)?\s*(\w+)", s)
+            if m:
+                uses.add(m.group(1))
+    return mods, uses
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--src", default="/root/reference/Src")
+    ap.add_argument("-j", type=int, default=8)
+    ap.add_argument("--no-exe", action="store_true")
+    a = ap.parse_args()
+    if not os.path.isdir(a.src):
+        print("reference sources not found at", a.src, "- skipping reference build")
+        return 0
+    os.makedirs(OUT, exist_ok=True)
+    srcs = sorted(f for f in os.listdir(a.src) if f.endswith(".f90") and f not in SKIP)
+    info = {f: scan(os.path.join(a.src, f)) for f in srcs}
+    provider = {}
+    for f, (mods, _) in info.items():
+        for m in mods:
+            provider[m] = f
+    deps = {f: {provider[u] for u in uses if u in provider and provider[u] != f}
+            for f, (_, uses) in info.items()}
+    # level-by-level topological compile (files within a level are independent)
+    done, levels = set(), []
+    while len(done) < len(srcs):
+        lvl = [f for f in srcs if f not in done and deps[f] <= done]
+        if not lvl:
+            raise SystemExit("dependency cycle among " + str(set(srcs) - done))
+        levels.append(lvl)
+        done |= set(lvl)
+
+    def obj(f):
+        return os.path.join(OUT, f[:-4] + ".o")
+
+    def compile_one(f):
+        src = os.path.join(a.src, f)
+        o = obj(f)
+        if os.path.exists(o) and os.path.getmtime(o) > os.path.getmtime(src):
+            return f, 0, ""
+        r = subprocess.run([FC, "-c", *FFLAGS, "-J", OUT, "-o", o, src],
+                           capture_output=True, text=True)
+        return f, r.returncode, r.stderr
+
+    with ThreadPoolExecutor(a.j) as ex:
+        for lvl in levels:
+            for f, rc, err in ex.map(compile_one, lvl):
+                if rc:
+                    sys.stderr.write(err)
+                    raise SystemExit("compile failed: " + f)
+    objs = [obj(f) for f in srcs]
+    lib_objs = [o for o in objs if not o.endswith("/main.o")]
+    if not a.no_exe:
+        subprocess.run([FC, "-fopenmp", "-o", os.path.join(OUT, "cassandra_ref.exe"), *objs],
+                       check=True)
+    probe_src = os.path.join(HERE, "ref_probe.f90")
+    if os.path.exists(probe_src):
+        po = os.path.join(OUT, "ref_probe.o")
+        subprocess.run([FC, "-c", *FFLAGS, "-I", OUT, "-J", OUT, "-o", po, probe_src], check=True)
+        subprocess.run([FC, "-fopenmp", "-o", os.path.join(OUT, "ref_probe"), po, *lib_objs],
+                       check=True)
+    print("reference build ok:", OUT)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,330 @@
+!------------------------------------------------------------------------------
+! ref_probe -- golden-vector probe over the Cassandra reference build.
+!
+! Test infrastructure only: this program is linked against the reference's own
+! objects in oracle/_ref (built by oracle/ref_build.py) and never ships in the
+! product.  It reads a Cassandra input file with the reference's own parser,
+! performs the same start-up calls the reference executable makes before its
+! first move (main.f90:215-400), and then calls reference subroutines directly,
+! printing every result with 17 significant digits.
+!
+!   ref_probe <input.inp> energy  <out>            energies of the start state
+!   ref_probe <input.inp> nvt     <out> <nsteps>   reference NVT move loop
+!   ref_probe <input.inp> widom   <out> <probe> <species>
+!                                                  Widom energies of placements
+!   ref_probe <input.inp> rng     <out> <ndraws>   rranf()/rranint() stream
+!------------------------------------------------------------------------------
+PROGRAM ref_probe
+  USE Global_Variables
+  USE File_Names
+  USE IO_Utilities
+  USE Input_Routines
+  USE Read_Write_Checkpoint
+  USE Energy_Routines
+  USE Random_Generators
+  USE Fragment_Growth
+  USE Internal_Coordinate_Routines
+  USE Simulation_Properties
+  IMPLICIT NONE
+
+  CHARACTER(LEN=240) :: inpname, mode, outname, arg4, arg5
+  INTEGER :: ios, ibox, is, im, lm, ou, nsteps, istep, ia, k, nplace, iplace
+  INTEGER :: wspecies
+  LOGICAL :: overlap, intra_ov
+  REAL(DP) :: evdw, eqq, e_intra_vdw, e_intra_qq, e_periodic
+  REAL(DP) :: rno
+  INTEGER(KIND=8) :: nsucc_before, nsucc_after
+
+  CALL GETARG(1, inpname)
+  CALL GETARG(2, mode)
+  CALL GETARG(3, outname)
+  ou = 77
+  OPEN(UNIT=ou, FILE=outname, ACTION='WRITE', STATUS='REPLACE')
+
+  IF (TRIM(mode) == 'rng') THEN
+     CALL GETARG(4, arg4)
+     READ(arg4, *) nsteps
+     CALL probe_rng(nsteps)
+     CLOSE(ou)
+     STOP
+  END IF
+
+  CALL start_up
+  ibox = 1
+  ! An nvt_mc input with read_config skips "# CBMC_Info" (input_routines.f90:6097),
+  ! leaving rcut_cbmc unset.  Set it to the 6.5 A that the generated inputs
+  ! declare, squared as the parser would (input_routines.f90:6084).
+  IF (rcut_cbmc(1) <= 0.0_DP) THEN
+     rcut_cbmc(:) = 6.5_DP
+     rcut_cbmcsq = rcut_cbmc * rcut_cbmc
+  END IF
+
+  SELECT CASE (TRIM(mode))
+  CASE ('energy')
+     CALL dump_energy('start')
+     DO is = 1, nspecies
+        DO im = 1, nmols(is, ibox)
+           lm = locate(im, is, ibox)
+           CALL Compute_Molecule_Nonbond_Inter_Energy(lm, is, evdw, eqq, overlap)
+           WRITE(ou, '(A,2I7,2ES26.17E3,L3)') 'mol ', is, lm, evdw, eqq, overlap
+        END DO
+     END DO
+     ! the same sums under the CBMC cutoff (cbmc_flag), fragment_growth.f90:192
+     CALL Set_CBMC_Flag(.TRUE.)
+     DO is = 1, nspecies
+        DO im = 1, nmols(is, ibox)
+           lm = locate(im, is, ibox)
+           CALL Compute_Molecule_Nonbond_Inter_Energy(lm, is, evdw, eqq, overlap)
+           WRITE(ou, '(A,2I7,2ES26.17E3,L3)') 'cbmc ', is, lm, evdw, eqq, overlap
+        END DO
+     END DO
+     CALL Set_CBMC_Flag(.FALSE.)
+     DO is = 1, nspecies
+        DO im = 1, MIN(nmols(is, ibox), 8)
+           lm = locate(im, is, ibox)
+           intra_ov = .FALSE.
+           CALL Compute_Molecule_Nonbond_Intra_Energy(lm, is, e_intra_vdw, e_intra_qq, &
+                e_periodic, intra_ov)
+           WRITE(ou, '(A,2I7,3ES26.17E3,L3)') 'intra ', is, lm, e_intra_vdw, e_intra_qq, &
+                e_periodic, intra_ov
+        END DO
+     END DO
+     IF (int_charge_sum_style(ibox) == charge_ewald) THEN
+        WRITE(ou, '(A,I8,2ES26.17E3)') 'ewald_setup ', nvecs(ibox), alpha_ewald(ibox), &
+             h_ewald_cut(ibox)
+        DO k = 1, nvecs(ibox)
+           WRITE(ou, '(A,I8,5ES26.17E3)') 'kvec ', k, hx(k,ibox), hy(k,ibox), hz(k,ibox), &
+                Cn(k,ibox), cos_sum(k,ibox)
+        END DO
+     END IF
+
+  CASE ('nvt')
+     CALL GETARG(4, arg4)
+     READ(arg4, *) nsteps
+     WRITE(ou, '(A,2ES26.17E3)') 'cuts ', cut_trans, cut_rot
+     CALL dump_energy('start')
+     DO istep = 1, nsteps
+        i_mcstep = istep
+        nsucc_before = total_successes()
+        rno = rranf()
+        IF (rno <= cut_trans) THEN
+           CALL Translate
+           nsucc_after = total_successes()
+           WRITE(ou, '(A,I8,L3,ES26.17E3,5I21)') 'T ', istep, nsucc_after > nsucc_before, &
+                energy(ibox)%total, s1, s2, s3, s4, s5
+        ELSE IF (rno <= cut_rot) THEN
+           CALL Rotate
+           nsucc_after = total_successes()
+           WRITE(ou, '(A,I8,L3,ES26.17E3,5I21)') 'R ', istep, nsucc_after > nsucc_before, &
+                energy(ibox)%total, s1, s2, s3, s4, s5
+        ELSE
+           WRITE(ou, '(A,I8)') 'X ', istep
+        END IF
+     END DO
+     ! incremental energy vs a full recompute, as Check_System_Energy does
+     CALL Compute_System_Total_Energy(ibox, .TRUE., overlap)
+     CALL dump_energy('end')
+
+  CASE ('widom')
+     CALL GETARG(4, arg4)
+     CALL GETARG(5, arg5)
+     READ(arg5, *) wspecies
+     CALL probe_widom(TRIM(arg4), wspecies)
+
+  CASE DEFAULT
+     WRITE(*,*) 'unknown mode ', TRIM(mode)
+     STOP 2
+  END SELECT
+  CLOSE(ou)
+
+CONTAINS
+
+  !----------------------------------------------------------------------------
+  SUBROUTINE probe_rng(n)
+    INTEGER, INTENT(IN) :: n
+    INTEGER :: i
+    INTEGER(KIND=8) :: a, b
+    REAL(DP) :: u
+    CHARACTER(LEN=64) :: sa, sb
+    CALL GETARG(5, sa)
+    CALL GETARG(6, sb)
+    READ(sa, *) a
+    READ(sb, *) b
+    CALL Init_Seeds(a, b)
+    DO i = 1, n
+       u = rranf()
+       WRITE(ou, '(ES26.17E3)') u
+    END DO
+    DO i = 1, 16
+       WRITE(ou, '(A,I21)') 'int ', rranint()
+    END DO
+  END SUBROUTINE probe_rng
+
+  !----------------------------------------------------------------------------
+  INTEGER(KIND=8) FUNCTION total_successes()
+    INTEGER :: js
+    total_successes = 0
+    DO js = 1, nspecies
+       total_successes = total_successes + nsuccess(js,1)%displacement + nsuccess(js,1)%rotation
+    END DO
+  END FUNCTION total_successes
+
+  !----------------------------------------------------------------------------
+  SUBROUTINE dump_energy(tag)
+    CHARACTER(LEN=*), INTENT(IN) :: tag
+    WRITE(ou, '(A,A,10ES26.17E3)') 'energy_', tag, energy(1)%total, energy(1)%intra, &
+         energy(1)%inter, energy(1)%inter_vdw, energy(1)%inter_q, energy(1)%lrc, &
+         energy(1)%reciprocal, energy(1)%self, energy(1)%intra_vdw, energy(1)%intra_q
+  END SUBROUTINE dump_energy
+
+  !----------------------------------------------------------------------------
+  ! Energies of probe placements as Widom_Insert evaluates them
+  ! (widom_insert.f90:180-240 prologue, :378-431 per insertion), without the
+  ! CBMC placement step: the probe coordinates come from a file, one
+  ! placement per block of natoms(is) lines.
+  SUBROUTINE probe_widom(fname, isp)
+    CHARACTER(LEN=*), INTENT(IN) :: fname
+    INTEGER, INTENT(IN) :: isp
+    INTEGER :: pu, i_type, i
+    INTEGER, ALLOCATABLE :: nbeads_in(:)
+    REAL(DP) :: e_lrc, lrc_diff, e_self, e_recip, e_inter, e_inter_cbmc
+    LOGICAL :: ov_inter, ov_cbmc, ov_intra
+
+    pu = 78
+    OPEN(UNIT=pu, FILE=fname, ACTION='READ')
+    READ(pu, *) nplace
+    ALLOCATE(nbeads_in(SIZE(nint_beads,1)))
+
+    nmols(isp,ibox) = nmols(isp,ibox) + 1
+    im = nmols(isp,ibox)
+    locate(im,isp,ibox) = locate(nmols(isp,0),isp,0)
+    locate(nmols(isp,0),isp,0) = 0
+    widom_species = isp
+    widom_locate = locate(im,isp,ibox)
+    molecule_list(widom_locate,isp)%which_box = ibox
+    molecule_list(widom_locate,isp)%frac = 1.0_DP
+    molecule_list(widom_locate,isp)%molecule_type = int_normal
+
+    lrc_diff = 0.0_DP
+    IF (int_vdw_sum_style(ibox) == vdw_cut_tail) THEN
+       nbeads_in = nint_beads(:,ibox)
+       DO i = 1, natoms(isp)
+          i_type = nonbond_list(i,isp)%atom_type_number
+          nint_beads(i_type,ibox) = nint_beads(i_type,ibox) + 1
+       END DO
+       CALL Compute_LR_correction(ibox, e_lrc)
+       nint_beads(:,ibox) = nbeads_in
+       lrc_diff = e_lrc - energy(ibox)%lrc
+    END IF
+    e_self = 0.0_DP
+    IF (int_charge_style(ibox) == charge_coul .AND. has_charge(isp)) THEN
+       CALL Compute_Molecule_Self_Energy(widom_locate, isp, ibox, e_self)
+    END IF
+    WRITE(ou, '(A,3ES26.17E3)') 'consts ', lrc_diff, e_self, energy(ibox)%reciprocal
+
+    widom_active = .TRUE.
+    ALLOCATE(widom_atoms(natoms(isp)))
+    widom_molecule = molecule_list(widom_locate,isp)
+    widom_atoms = atom_list(1:natoms(isp),widom_locate,isp)
+    widom_molecule%live = .TRUE.
+    widom_atoms%exist = .TRUE.
+
+    DO iplace = 1, nplace
+       DO ia = 1, natoms(isp)
+          READ(pu, *) widom_atoms(ia)%rxp, widom_atoms(ia)%ryp, widom_atoms(ia)%rzp
+       END DO
+       CALL Get_COM(widom_locate, isp)
+       CALL Compute_Max_Com_Distance(widom_locate, isp)
+       ! full-cutoff energy (widom_insert.f90:389-399)
+       CALL Compute_Molecule_Nonbond_Inter_Energy_Widom(widom_locate, isp, e_inter, ov_inter)
+       e_intra_vdw = 0.0_DP
+       e_intra_qq = 0.0_DP
+       e_periodic = 0.0_DP
+       ov_intra = .FALSE.
+       IF (.NOT. ov_inter) THEN
+          CALL Compute_Molecule_Nonbond_Intra_Energy(widom_locate, isp, e_intra_vdw, &
+               e_intra_qq, e_periodic, ov_intra)
+       END IF
+       e_recip = 0.0_DP
+       IF (int_charge_sum_style(ibox) == charge_ewald .AND. has_charge(isp)) THEN
+          CALL Update_System_Ewald_Reciprocal_Energy_Widom(widom_locate, isp, ibox, e_recip)
+       END IF
+       ! CBMC-cutoff trial energy (fragment_growth.f90:519-526)
+       CALL Set_CBMC_Flag(.TRUE.)
+       CALL Compute_Molecule_Nonbond_Inter_Energy_Widom(widom_locate, isp, e_inter_cbmc, ov_cbmc)
+       CALL Set_CBMC_Flag(.FALSE.)
+       WRITE(ou, '(A,I8,6ES26.17E3,2L3)') 'place ', iplace, e_inter, e_periodic, &
+            e_intra_vdw, e_intra_qq, e_recip, e_inter_cbmc, ov_inter, ov_cbmc
+    END DO
+    CLOSE(pu)
+  END SUBROUTINE probe_widom
+
+  !----------------------------------------------------------------------------
+  ! The reference executable's start-up sequence up to its first energy
+  ! evaluation (main.f90:130-400), for nvt_mc inputs with read_config starts.
+  SUBROUTINE start_up
+    INTEGER :: openstatus, jbox, js, jm, jlm
+    inputfile = inpname
+    OPEN(UNIT=inputunit, FILE=inputfile, STATUS='OLD', IOSTAT=openstatus, ACTION='READ')
+    IF (openstatus /= 0) STOP 'cannot open input'
+    CALL Get_Run_Name
+    CALL Name_Files(run_name, '.log', logfile)
+    OPEN(UNIT=logunit, FILE=logfile)
+    CALL Get_Verbosity_Info
+    need_solvents = .FALSE.
+    precalc_atompair_nrg = .FALSE.
+    CALL Get_Sim_Type
+    calc_rmin_flag = .FALSE.
+    IF (int_sim_type /= sim_nvt) STOP 'ref_probe handles nvt_mc inputs only'
+    CALL NVTMC_Control
+    CALL Get_Run_Type
+    CALL Initialize
+    CALL Init_Seeds(iseed1, iseed3)
+    molecule_list(:,:)%live = .FALSE.
+    molecule_list(:,:)%which_box = 0
+    atom_list(:,:,:)%exist = .FALSE.
+    molecule_list(:,:)%frac = 0.0_DP
+    cbmc_flag = .FALSE.
+    DO jbox = 1, nbr_boxes
+       IF (start_type(jbox) /= 'read_config') STOP 'ref_probe needs read_config'
+       CALL Read_Config(jbox)
+    END DO
+    DO js = 1, nspecies
+       DO jm = max_molecules(js), SUM(nmols(js,1:nbr_boxes)) + 1, -1
+          nmols(js,0) = nmols(js,0) + 1
+          locate(nmols(js,0),js,0) = jm
+       END DO
+    END DO
+    IF (int_charge_sum_style(1) == charge_ewald) THEN
+       ALLOCATE(hx(maxk,nbr_boxes), hy(maxk,nbr_boxes), hz(maxk,nbr_boxes), &
+            hsq(maxk,nbr_boxes), Cn(maxk,nbr_boxes))
+       ALLOCATE(nvecs(nbr_boxes))
+       DO jbox = 1, nbr_boxes
+          CALL Ewald_Reciprocal_Lattice_Vector_Setup(jbox)
+       END DO
+       ALLOCATE(cos_sum(MAXVAL(nvecs),nbr_boxes), sin_sum(MAXVAL(nvecs),nbr_boxes))
+       ALLOCATE(cos_sum_old(MAXVAL(nvecs),nbr_boxes), sin_sum_old(MAXVAL(nvecs),nbr_boxes))
+       ALLOCATE(cos_sum_start(MAXVAL(nvecs),nbr_boxes), sin_sum_start(MAXVAL(nvecs),nbr_boxes))
+       ALLOCATE(cos_mol(MAXVAL(nvecs),SUM(max_molecules)), sin_mol(MAXVAL(nvecs),SUM(max_molecules)))
+       cos_mol(:,:) = 0.0_DP
+       sin_mol(:,:) = 0.0_DP
+    END IF
+    CALL Get_Internal_Coords
+    DO jbox = 1, nbr_boxes
+       DO js = 1, nspecies
+          DO jm = 1, nmols(js,jbox)
+             jlm = locate(jm,js,jbox)
+             CALL Get_COM(jlm,js)
+             CALL Compute_Max_Com_Distance(jlm,js)
+             CALL Fold_Molecule(jlm,js,jbox)
+          END DO
+       END DO
+    END DO
+    CALL Init_Accumulators
+    DO jbox = 1, nbr_boxes
+       CALL Compute_System_Total_Energy(jbox, .TRUE., overlap)
+       IF (overlap) STOP 'overlap in the start configuration'
+    END DO
+  END SUBROUTINE start_up
+
+END PROGRAM ref_probe

@@ -1,0 +1,51 @@
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libcassgpu.so on cuda:0)")
+
+
+def golden_json(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def golden_npz(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+def style_systems():
+    """The style cases of tests/golden/make_golden.py, rebuilt from their seeds."""
+    from tests.golden.cases import style_cases
+    return {s.name: s for s in style_cases()}
+
+
+def config_by_name(name):
+    from cassandra_amd import systems as S
+    table = {"lj500": S.config_lj500, "spce512": S.config_spce512, "lj2000": S.config_lj2000,
+             "co2n2": S.config_co2n2}
+    if name in table:
+        return table[name]()
+    return style_systems()[name]
+
+
+def mol_index(system, species_1based, locate_1based):
+    """Global molecule index of the reference's (is, locate) pair."""
+    return int(sum(system.nmols[:species_1based - 1]) + locate_1based - 1)
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    from oracle import oracle
+    oracle.build()
+    return oracle

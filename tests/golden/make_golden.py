@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Regenerate the golden fixtures in tests/golden/ from the reference build.
+
+Runs only where the reference sources exist (they are compiled into
+oracle/_ref by oracle/ref_build.py).  Each case is written as Cassandra input
+files (cassandra_amd.formats), evaluated by oracle/_ref/ref_probe -- which
+calls the reference's own subroutines -- and the printed results are stored
+as small JSON / NPZ fixtures.  The fixtures are data: inputs and the
+reference's outputs.
+
+Usage: python3 tests/golden/make_golden.py [case ...]
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from cassandra_amd import formats, systems  # noqa: E402
+from tests.golden.cases import style_cases  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+PROBE = os.path.join(REF, "ref_probe")
+RUNS = os.path.join(REF, "runs")
+ENV = dict(os.environ, OMP_NUM_THREADS="1")
+
+
+def run_probe(case_dir, inp, *args):
+    out = os.path.join(case_dir, args[0] + ".txt")
+    subprocess.run([PROBE, os.path.basename(inp), args[0], os.path.basename(out), *map(str, args[1:])],
+                   cwd=case_dir, check=True, env=ENV, stdout=subprocess.DEVNULL)
+    with open(out) as f:
+        return f.read().splitlines()
+
+
+def f64(tok):
+    return float(tok.replace("D", "E"))
+
+
+def parse_energy(lines):
+    res = {"mol": [], "cbmc": [], "intra": [], "kvec": []}
+    for ln in lines:
+        t = ln.split()
+        if not t:
+            continue
+        if t[0].startswith("energy_"):
+            res[t[0]] = [f64(x) for x in t[1:]]
+        elif t[0] in ("mol", "cbmc"):
+            res[t[0]].append([int(t[1]), int(t[2]), f64(t[3]), f64(t[4]), t[5] == "T"])
+        elif t[0] == "intra":
+            res["intra"].append([int(t[1]), int(t[2]), f64(t[3]), f64(t[4]), f64(t[5]), t[6] == "T"])
+        elif t[0] == "ewald_setup":
+            res["ewald_setup"] = [int(t[1]), f64(t[2]), f64(t[3])]
+        elif t[0] == "kvec":
+            res["kvec"].append([f64(x) for x in t[2:]])
+    return res
+
+
+ENERGY_FIELDS = ["total", "intra", "inter", "inter_vdw", "inter_q", "lrc", "reciprocal", "self",
+                 "intra_vdw", "intra_q"]
+
+
+def energy_fixture(name, system):
+    d = os.path.join(RUNS, name)
+    inp = formats.write_case(system, d, name)
+    r = parse_energy(run_probe(d, inp, "energy"))
+    fx = {"case": name, "energy": dict(zip(ENERGY_FIELDS, r["energy_start"])),
+          "mol": [[m[0], m[1], repr(m[2]), repr(m[3]), m[4]] for m in r["mol"]],
+          "cbmc": [[m[0], m[1], repr(m[2]), repr(m[3]), m[4]] for m in r["cbmc"]],
+          "intra": [[m[0], m[1], repr(m[2]), repr(m[3]), repr(m[4]), m[5]] for m in r["intra"]]}
+    fx["energy"] = {k: repr(v) for k, v in fx["energy"].items()}
+    if "ewald_setup" in r:
+        kv = np.array(r["kvec"])
+        fx["ewald"] = {"nvecs": r["ewald_setup"][0], "alpha": repr(r["ewald_setup"][1]),
+                       "h_cut": repr(r["ewald_setup"][2]),
+                       "head": [[repr(float(x)) for x in row] for row in kv[:24]],
+                       "tail": [[repr(float(x)) for x in row] for row in kv[-24:]],
+                       "sum_cn": repr(float(kv[:, 3].sum())),
+                       "sum_cn_s2": repr(float((kv[:, 3] * kv[:, 4] ** 2).sum()))}
+    with open(os.path.join(HERE, f"{name}_energy.json"), "w") as f:
+        json.dump(fx, f, indent=0)
+    formats.write_xyz(system, os.path.join(HERE, f"{name}_start.xyz"))
+    print(name, "energy: total", fx["energy"]["total"])
+
+
+def nvt_fixture(name, system, nsteps):
+    d = os.path.join(RUNS, name)
+    inp = formats.write_case(system, d, name)
+    lines = run_probe(d, inp, "nvt", nsteps)
+    kind, acc, etot, rng = [], [], [], []
+    e0 = e1 = None
+    for ln in lines:
+        t = ln.split()
+        if t[0] in ("T", "R"):
+            kind.append(0 if t[0] == "T" else 1)
+            acc.append(t[2] == "T")
+            etot.append(f64(t[3]))
+            rng.append([int(x) for x in t[4:9]])
+        elif t[0] == "energy_start":
+            e0 = [f64(x) for x in t[1:]]
+        elif t[0] == "energy_end":
+            e1 = [f64(x) for x in t[1:]]
+        elif t[0] == "X":
+            raise SystemExit("unexpected move kind in " + name)
+    kind = np.array(kind, dtype=np.uint8)
+    acc = np.array(acc, dtype=bool)
+    etot = np.array(etot)
+    rng = np.array(rng, dtype=np.int64)
+    np.savez_compressed(os.path.join(HERE, f"{name}_nvt.npz"),
+                        nsteps=np.int64(nsteps), kind=np.packbits(kind), accept=np.packbits(acc),
+                        energy_every100=etot[99::100], rng_every1000=rng[999::1000],
+                        energy_incremental_end=etot[-1], energy_start=np.array(e0),
+                        energy_recomputed_end=np.array(e1))
+    print(name, f"nvt: {nsteps} moves, accepted {int(acc.sum())}")
+
+
+def probe_placements(frame: systems.System, probe: systems.Species, n, seed, n_near=20):
+    """Uniform COM positions and orientations in the frame; the last n_near
+    placements put a probe site 0.5-1.1 A from a frame site, to exercise the
+    overlap path (r < rcut_low)."""
+    rng = np.random.default_rng(seed)
+    L = frame.box
+    ref = probe.ref_coords - systems.com_of(probe, probe.ref_coords)
+    frame_xyz = np.concatenate([c.reshape(-1, 3) for c in frame.coords if c.size])
+    out = np.zeros((n, probe.natoms, 3))
+    for i in range(n):
+        R = systems.random_rotation(rng) if probe.natoms > 1 else np.eye(3)
+        if i >= n - n_near:
+            host = frame_xyz[rng.integers(len(frame_xyz))]
+            v = rng.normal(size=3)
+            v /= np.linalg.norm(v)
+            c = host + v * rng.uniform(0.5, 1.1) - (ref @ R.T)[0]
+        else:
+            c = (rng.random(3) - 0.5) * L
+        xyz = ref @ R.T + c
+        # keep the COM inside the central box as Build_Molecule places it
+        out[i] = xyz
+    return out
+
+
+def widom_fixture(name, frame, probe, n=400, seed=7):
+    system = systems.with_probe(frame, probe)
+    d = os.path.join(RUNS, name)
+    inp = formats.write_case(system, d, name)
+    place = probe_placements(frame, probe, n, seed)
+    pfile = os.path.join(d, "placements.txt")
+    with open(pfile, "w") as f:
+        f.write(f"{n}\n")
+        for p in place:
+            for x in p:
+                f.write(" ".join(repr(float(v)) for v in x) + "\n")
+    lines = run_probe(d, inp, "widom", "placements.txt", len(system.species))
+    consts, rows, flags = None, [], []
+    for ln in lines:
+        t = ln.split()
+        if t[0] == "consts":
+            consts = [f64(x) for x in t[1:4]]
+        elif t[0] == "place":
+            rows.append([f64(x) for x in t[2:8]])
+            flags.append([t[8] == "T", t[9] == "T"])
+    np.savez_compressed(os.path.join(HERE, f"{name}_widom.npz"), placements=place,
+                        consts=np.array(consts), energies=np.array(rows), overlap=np.array(flags))
+    print(name, f"widom: {n} placements, overlaps {int(np.array(flags)[:, 0].sum())}")
+
+
+def rng_fixture():
+    cases = []
+    for seeds in [(1234567, 7654321), (21498, 489625), (1, 2), (0, 4095)]:
+        out = os.path.join(RUNS, "rng.txt")
+        os.makedirs(RUNS, exist_ok=True)
+        subprocess.run([PROBE, "none", "rng", out, "200", str(seeds[0]), str(seeds[1])],
+                       check=True, env=ENV, stdout=subprocess.DEVNULL)
+        with open(out) as f:
+            lines = f.read().splitlines()
+        draws = [repr(f64(x)) for x in lines if not x.startswith("int")]
+        ints = [int(x.split()[1]) for x in lines if x.startswith("int")]
+        cases.append({"seeds": list(seeds), "draws": draws, "ints": ints})
+    with open(os.path.join(HERE, "rng.json"), "w") as f:
+        json.dump(cases, f, indent=0)
+    print("rng:", len(cases), "seed pairs")
+
+
+def main(which):
+    if not os.path.exists(PROBE):
+        raise SystemExit("build the reference first: python3 oracle/ref_build.py")
+    want = lambda k: not which or k in which  # noqa: E731
+    if want("rng"):
+        rng_fixture()
+    lj500 = systems.config_lj500()
+    spce = systems.config_spce512()
+    if want("lj500"):
+        energy_fixture("lj500", lj500)
+        nvt_fixture("lj500", lj500, 100000)
+    if want("spce512"):
+        energy_fixture("spce512", spce)
+        nvt_fixture("spce512", spce, 100000)
+    if want("widom"):
+        lj2000 = systems.config_lj2000()
+        energy_fixture("lj2000", lj2000)
+        widom_fixture("lj2000_ch4", lj2000, systems.species_methane())
+        widom_fixture("lj2000_co2", lj2000, systems.species_co2())
+        widom_fixture("spce512_co2", spce, systems.species_co2())
+        widom_fixture("spce512_ch4", spce, systems.species_methane())
+    if want("co2n2"):
+        energy_fixture("co2n2", systems.config_co2n2())
+    if want("styles"):
+        for s in style_cases():
+            energy_fixture(s.name, s)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

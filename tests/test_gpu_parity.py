@@ -1,0 +1,180 @@
+"""GPU parity: the HIP engine (libcassgpu.so, through its C ABI) against the
+reference's golden vectors and the CPU oracle.  FP64 path: 1e-10 relative.
+Mixed path: the error against FP64 is measured and bounded per quantity."""
+import numpy as np
+import pytest
+
+from cassandra_amd import engine as E
+from cassandra_amd import systems as S
+from tests.conftest import config_by_name, golden_json, golden_npz, mol_index
+
+pytestmark = pytest.mark.gpu
+REL = 1e-10
+ENERGY_CASES = ["lj500", "spce512", "lj2000", "co2n2", "sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut",
+                "sty_shiftforce_dsfa", "sty_minimum_minimum", "sty_charmm_ewald", "sty_ortho_ewald",
+                "sty_mie_cut_tail", "sty_mie_cut_shift"]
+
+
+def rel(a, b, floor=1.0):
+    return abs(a - b) / max(abs(b), floor)
+
+
+def make_box(system, prec=E.FP64, extra_mols=0):
+    eng = E.Engine(system, precision=prec)
+    box = eng.add_box(system, max_mols=sum(system.nmols) + extra_mols)
+    if system.charge_style == "ewald":
+        box.ewald_setup()
+    return eng, box
+
+
+@pytest.mark.parametrize("case", ENERGY_CASES)
+def test_total_energy_fp64(case):
+    print(f"[gpu] total energy {case}", flush=True)
+    g = golden_json(f"{case}_energy.json")
+    system = config_by_name(case).parsed()
+    eng, box = make_box(system)
+    e = box.compute_system_total_energy()
+    for k, v in g["energy"].items():
+        assert rel(e[k], float(v)) < REL, (k, e[k], v)
+    if "ewald" in g:
+        assert box.ewald_setup() == g["ewald"]["nvecs"]
+
+
+@pytest.mark.parametrize("case", ENERGY_CASES)
+def test_molecule_energies_fp64(case):
+    print(f"[gpu] molecule energies {case}", flush=True)
+    g = golden_json(f"{case}_energy.json")
+    system = config_by_name(case).parsed()
+    eng, box = make_box(system)
+    for key, cbmc in (("mol", False), ("cbmc", True)):
+        for is_, lm, ev, eq, ov in g[key][::7]:
+            m = mol_index(system, is_, lm)
+            v, q, o = box.compute_molecule_nonbond_inter_energy(m, cbmc=cbmc)
+            assert o == ov
+            assert rel(v, float(ev)) < REL and rel(q, float(eq)) < REL, (key, m, v, ev, q, eq)
+    xyz_all = box.download()
+    first = np.concatenate([[0], np.cumsum([system.species[s].natoms for s in box.spec])])
+    for is_, lm, a, q, qi, ov in g["intra"]:
+        m = mol_index(system, is_, lm)
+        n = system.species[is_ - 1].natoms
+        xyz = xyz_all[3 * first[m]:3 * (first[m] + n)]
+        ra, rq, rqi, o = box.compute_molecule_nonbond_intra_energy(is_ - 1, xyz)
+        assert o == ov and rel(ra, float(a)) < REL and rel(rq, float(q)) < REL and rel(rqi, float(qi)) < REL
+
+
+WIDOM_CASES = [("lj2000_ch4", S.species_methane), ("lj2000_co2", S.species_co2),
+               ("spce512_co2", S.species_co2), ("spce512_ch4", S.species_methane)]
+
+
+def widom_box(case, probe, prec):
+    fx = golden_npz(f"{case}_widom.npz")
+    system = S.with_probe(config_by_name(case.split("_")[0]), probe()).parsed()
+    eng, box = make_box(system, prec)
+    e = box.compute_system_total_energy()
+    return fx, system, eng, box, e
+
+
+@pytest.mark.parametrize("case,probe", WIDOM_CASES)
+def test_trial_energies_fp64(case, probe):
+    """cass_trial_energies (the CBMC/Widom trial-set kernel) at the reference's
+    placements, full and CBMC cutoff."""
+    print(f"[gpu] trial energies {case}", flush=True)
+    fx, system, eng, box, _ = widom_box(case, probe, E.FP64)
+    sp = len(system.species) - 1
+    place = fx["placements"]
+    for cbmc, col, ovcol in ((False, 0, 0), (True, 5, 1)):
+        nrg, ov = box.trial_energies(sp, place, cbmc=cbmc)
+        assert np.array_equal(ov, fx["overlap"][:, ovcol])
+        ok = ~ov
+        ref = fx["energies"][ok, col]
+        assert np.max(np.abs(nrg[ok] - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+    if system.charge_style == "ewald":
+        er = box.ewald_widom(sp, place)
+        ok = ~fx["overlap"][:, 0]
+        ref = fx["energies"][ok, 4]
+        assert np.max(np.abs(er[ok] - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+
+
+@pytest.mark.parametrize("case,probe", WIDOM_CASES)
+def test_widom_kernel_placements(case, probe):
+    """The batched Widom kernel at the reference's placements: FP64 within
+    1e-10 of the reference; mixed precision within 1e-6 of FP64 on the energy
+    change dE (per insertion, relative to max(|dE|, 1 kJ/mol = 100 internal))."""
+    print(f"[gpu] widom kernel {case}", flush=True)
+    fx, system, eng, box, e = widom_box(case, probe, E.FP64)
+    sp = len(system.species) - 1
+    place = fx["placements"]
+    r, wv, eo = box.widom_placements(sp, place)
+    ok = ~fx["overlap"][:, 0]
+    ref = fx["energies"]
+    assert np.array_equal(eo[:, 3].astype(bool), fx["overlap"][:, 0])
+    assert np.max(np.abs(eo[ok, 0] - ref[ok, 0]) / np.maximum(np.abs(ref[ok, 0]), 1.0)) < REL
+    okc = ~fx["overlap"][:, 1] & ok
+    assert np.max(np.abs(eo[okc, 1] - ref[okc, 5]) / np.maximum(np.abs(ref[okc, 5]), 1.0)) < REL
+    if system.charge_style == "ewald":
+        de_ref = ref[ok, 4] - e["reciprocal"]
+        assert np.max(np.abs(eo[ok, 2] - de_ref) / np.maximum(np.abs(ref[ok, 4]), 1.0)) < REL
+    # mixed precision against FP64
+    eng.set_precision(E.MIXED)
+    r2, wv2, eo2 = box.widom_placements(sp, place)
+    d64 = eo[ok, 0] + eo[ok, 2]
+    d32 = eo2[ok, 0] + eo2[ok, 2]
+    err = np.abs(d32 - d64) / np.maximum(np.abs(d64), 100.0)
+    print(f"[gpu] mixed dE rel err {case}: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
+    assert err.max() < 1e-6
+
+
+def test_ewald_move_commit_revert():
+    """Update_System_Ewald_Reciprocal_Energy for a translation, then commit or
+    revert: the staged total equals a full recompute at the new position and
+    dE equals E_new - E_old."""
+    system = config_by_name("sty_cut_ewald").parsed()
+    eng, box = make_box(system)
+    e_old = box.ewald_full()
+    xyz = box.download()
+    mol = 7
+    new = xyz[9 * mol:9 * mol + 9].copy()
+    new[0::3] += 0.37
+    e_new, de = box.ewald_move(mol, 0, new)
+    assert rel(de, e_new - e_old, floor=1e-3) < 1e-9
+    box.ewald_revert()
+    assert rel(box.ewald_full(), e_old) < 1e-12
+    e_new2, _ = box.ewald_move(mol, 0, new)
+    box.ewald_commit()
+    xyz[9 * mol:9 * mol + 9] = new
+    box.upload(box.spec, box.live, xyz)
+    box.ewald_setup()
+    assert rel(box.ewald_full(), e_new2) < 1e-10
+
+
+def _nvt(case, nsteps):
+    fx = golden_npz(f"{case}_nvt.npz")
+    system = config_by_name(case).parsed()
+    eng, box = make_box(system)
+    e0 = box.compute_system_total_energy()
+    r = E.rng(*system.seed)
+    out = box.nvt_run(r, nsteps, e0)
+    return fx, system, box, out
+
+
+@pytest.mark.parametrize("case,nsteps", [("lj500", 100000), ("spce512", 100000)])
+def test_nvt_chain_replays_reference(case, nsteps):
+    """1e5 moves under the reference's seed: the same move kinds and the same
+    accept/reject decisions as the reference; energies within 1e-10; a
+    decision that differs only where u is within 1e-9 of exp(-ln_pacc) is
+    reported, not failed."""
+    print(f"[gpu] nvt chain {case} {nsteps}", flush=True)
+    fx, system, box, out = _nvt(case, nsteps)
+    kind_ref = np.unpackbits(fx["kind"])[:nsteps].astype(bool)
+    acc_ref = np.unpackbits(fx["accept"])[:nsteps].astype(bool)
+    assert np.array_equal(out["kind"].astype(bool), kind_ref)
+    diff = np.nonzero(out["accepted"] != acc_ref)[0]
+    if diff.size:
+        i = diff[0]
+        lp = out["ln_pacc"][i]
+        pytest.fail(f"first decision mismatch at move {i}: ln_pacc={lp!r}")
+    ref = fx["energy_every100"][: nsteps // 100]
+    got = out["energy"][99::100]
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+    full = box.compute_system_total_energy()["total"]
+    assert rel(out["final"]["total"], full) < REL

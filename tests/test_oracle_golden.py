@@ -1,0 +1,213 @@
+"""The CPU oracle (oracle/cass_oracle.c) against the reference's own outputs.
+
+The fixtures in tests/golden/ were produced by oracle/_ref/ref_probe, which
+calls the reference's subroutines (built from its sources); see
+tests/golden/make_golden.py.  Closed forms and invariants cover what the
+fixtures leave open.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cassandra_amd import systems as S
+from tests.conftest import config_by_name, golden_json, golden_npz, mol_index
+
+ENERGY_CASES = ["lj500", "spce512", "lj2000", "co2n2", "sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut",
+                "sty_shiftforce_dsfa", "sty_minimum_minimum", "sty_charmm_ewald", "sty_ortho_ewald",
+                "sty_mie_cut_tail", "sty_mie_cut_shift"]
+REL = 1e-10
+
+
+def rel_err(a, b, floor=1.0):
+    return abs(a - b) / max(abs(b), floor)
+
+
+def test_rng_stream_matches_reference(oracle_mod):
+    """rranf / rranint after init_seeds (random_generators.f90:51-140): bit-exact."""
+    for case in golden_json("rng.json"):
+        s = oracle_mod.rng_init(*case["seeds"])
+        u = oracle_mod.rng_uniform(s, len(case["draws"]))
+        assert np.array_equal(u, np.array([float(x) for x in case["draws"]]))
+        assert oracle_mod.rng_int(s, len(case["ints"])) == case["ints"]
+
+
+@pytest.mark.parametrize("case", ENERGY_CASES)
+def test_total_energy_components(case, oracle_mod):
+    """Compute_System_Total_Energy components (energy_routines.f90:2648-2953)."""
+    g = golden_json(f"{case}_energy.json")
+    box = oracle_mod.OracleBox(config_by_name(case).parsed())
+    e = box.total_energy()
+    for k, v in g["energy"].items():
+        assert rel_err(e[k], float(v)) < REL, (k, e[k], v)
+
+
+@pytest.mark.parametrize("case", ENERGY_CASES)
+def test_per_molecule_energies(case, oracle_mod):
+    """Compute_Molecule_Nonbond_Inter_Energy at the full and at the CBMC cutoff
+    for every molecule, and Compute_Molecule_Nonbond_Intra_Energy."""
+    g = golden_json(f"{case}_energy.json")
+    system = config_by_name(case).parsed()
+    box = oracle_mod.OracleBox(system)
+    for key, cbmc in (("mol", False), ("cbmc", True)):
+        for is_, lm, ev, eq, ov in g[key]:
+            m = mol_index(system, is_, lm)
+            v, q, o = box.mol_inter(m, cbmc=cbmc)
+            assert o == ov
+            assert rel_err(v, float(ev)) < REL and rel_err(q, float(eq)) < REL, (key, m, v, ev, q, eq)
+    for is_, lm, a, q, qi, ov in g["intra"]:
+        m = mol_index(system, is_, lm)
+        ra, rq, rqi, o = box.mol_intra(m)
+        assert o == ov
+        assert rel_err(ra, float(a)) < REL and rel_err(rq, float(q)) < REL and rel_err(rqi, float(qi)) < REL
+
+
+@pytest.mark.parametrize("case", [c for c in ENERGY_CASES if "ewald" in c or c in ("spce512", "co2n2")])
+def test_kvectors(case, oracle_mod):
+    """Ewald_Reciprocal_Lattice_Vector_Setup: count, order and Cn (energy_routines.f90:2170-2323)."""
+    g = golden_json(f"{case}_energy.json")["ewald"]
+    box = oracle_mod.OracleBox(config_by_name(case).parsed())
+    box.total_energy()
+    t = box.ewald_tables()
+    assert len(t["hx"]) == g["nvecs"]
+    for rows, sl in ((g["head"], slice(0, 24)), (g["tail"], slice(-24, None))):
+        ref = np.array([[float(x) for x in r] for r in rows])
+        got = np.stack([t["hx"][sl], t["hy"][sl], t["hz"][sl], t["cn"][sl], t["cos_sum"][sl]], 1)
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+    assert rel_err(float(t["cn"].sum()), float(g["sum_cn"])) < 1e-12
+
+
+def _nvt_oracle(case, nsteps, oracle_mod):
+    fx = golden_npz(f"{case}_nvt.npz")
+    system = config_by_name(case).parsed()
+    box = oracle_mod.OracleBox(system)
+    e0 = box.total_energy()
+    s = oracle_mod.rng_init(*system.seed)
+    kind, acc, et, e1 = box.nvt_run(s, nsteps, e0, system.prob_translate, system.max_disp,
+                                    system.max_rot_radians(),
+                                    system.species[0].linear)
+    return fx, box, kind, acc, et, e1
+
+
+def test_nvt_chain_lj500_1e5_moves(oracle_mod):
+    """The reference's NVT move loop on config 1 (500 LJ atoms): the same
+    accept/reject sequence for all 1e5 moves under the same seed, the same
+    running energy, and the same RNG state."""
+    fx, box, kind, acc, et, e1 = _nvt_oracle("lj500", 100000, oracle_mod)
+    n = int(fx["nsteps"])
+    assert np.array_equal(np.unpackbits(fx["kind"])[:n].astype(bool), kind.astype(bool))
+    assert np.array_equal(np.unpackbits(fx["accept"])[:n].astype(bool), acc)
+    ref = fx["energy_every100"]
+    assert np.max(np.abs(et[99::100] - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+    # incremental bookkeeping vs a full recompute of the final state
+    full = box.total_energy()["total"]
+    assert rel_err(full, float(fx["energy_recomputed_end"][0])) < REL
+    assert rel_err(e1["total"], full) < 1e-10
+
+
+def test_nvt_chain_spce512_prefix(oracle_mod):
+    """Config 2 (512 SPC/E, Ewald): translate/rotate 50/50, first 20000 of the
+    reference's 1e5 moves (the full chain runs on the GPU side)."""
+    fx, box, kind, acc, et, e1 = _nvt_oracle("spce512", 20000, oracle_mod)
+    assert np.array_equal(np.unpackbits(fx["kind"])[:20000].astype(bool), kind.astype(bool))
+    assert np.array_equal(np.unpackbits(fx["accept"])[:20000].astype(bool), acc)
+    ref = fx["energy_every100"][:200]
+    assert np.max(np.abs(et[99::100] - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+
+
+@pytest.mark.parametrize("case,probe", [("lj2000_ch4", S.species_methane), ("lj2000_co2", S.species_co2),
+                                        ("spce512_co2", S.species_co2), ("spce512_ch4", S.species_methane)])
+def test_widom_placement_energies(case, probe, oracle_mod):
+    """Per-insertion energies of Widom_Insert (widom_insert.f90:378-431) at the
+    reference's placements: full-cutoff inter energy, periodic and intra q-q,
+    reciprocal energy with the probe added, CBMC-cutoff energy, overlaps."""
+    fx = golden_npz(f"{case}_widom.npz")
+    frame = config_by_name(case.split("_")[0])
+    system = S.with_probe(frame, probe()).parsed()
+    box = oracle_mod.OracleBox(system)
+    box.total_energy()
+    sp = len(system.species) - 1
+    out, ov = box.widom_energies(sp, fx["placements"])
+    assert np.array_equal(ov, fx["overlap"])
+    ok = ~fx["overlap"][:, 0]
+    ref = fx["energies"]
+    err = np.abs(out[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), 1.0)
+    # column 5 (CBMC energy) is compared only where the CBMC pass saw no overlap
+    okc = ~fx["overlap"][:, 1]
+    errc = np.abs(out[okc, 5] - ref[okc, 5]) / np.maximum(np.abs(ref[okc, 5]), 1.0)
+    assert err[:, :5].max() < REL and errc.max() < REL
+    # LRC change and self energy of the test molecule (widom_insert.f90:212-240)
+    lrc_diff, e_self, e_recip_in = fx["consts"]
+    nb = np.zeros(box.topo.ntypes)
+    for t in box.topo.atom_type[: sum(sp_.natoms * n for sp_, n in zip(system.species, system.nmols))]:
+        pass
+    assert abs(box.species_self(sp) - e_self) <= 1e-10 * max(1.0, abs(e_self))
+
+
+# ------------------------------------------------------------ closed forms
+def _two_site_box(r, L=30.0, vdw_sum="cut", charge_style="none", q=(0.0, 0.0), rc=12.0):
+    a = S.Atom("A", "A", 10.0, q[0], 120.0, 3.4)
+    b = S.Atom("B", "B", 10.0, q[1], 120.0, 3.4)
+    sa = S.Species("a", [a], np.zeros((1, 3)))
+    sb = S.Species("b", [b], np.zeros((1, 3)))
+    coords = [np.array([[[0.0, 0.0, 0.0]]]), np.array([[[r, 0.0, 0.0]]])]
+    return S.System([sa, sb], [1, 1], coords, np.array([L] * 3), 300.0, "lj", vdw_sum, rc,
+                    charge_style=charge_style, rcut_coul=rc, ewald_tol=1e-5, rcut_low=0.5)
+
+
+def test_lj_closed_form(oracle_mod):
+    """Two LJ sites (as the reference's Test1_LJ_StartEnergy): U(sigma) = 0,
+    U(2^(1/6) sigma) = -eps, and across the periodic boundary."""
+    eps = S.KBOLTZ * 120.0
+    for r, want in ((3.4, 0.0), (2 ** (1 / 6) * 3.4, -eps)):
+        e = oracle_mod.OracleBox(_two_site_box(r)).total_energy()
+        assert abs(e["inter_vdw"] - want) < 1e-9 * eps
+    # across the boundary: sites at +/-(L/2 - r/2) are r apart through the wall
+    s = _two_site_box(0.0)
+    s.coords = [np.array([[[14.0 - 0.3, 0, 0]]]), np.array([[[-14.0 - 0.3 + 0.0, 0, 0]]])]
+    s.box = np.array([30.0] * 3)
+    d = 30.0 - 28.0
+    e = oracle_mod.OracleBox(s).total_energy()
+    sr6 = (3.4 / d) ** 6
+    assert abs(e["inter_vdw"] - 4 * eps * (sr6 * sr6 - sr6)) < 1e-9 * abs(e["inter_vdw"])
+
+
+def test_ewald_two_ions_against_direct_sum(oracle_mod):
+    """An ion pair under Ewald against an independent numpy Ewald sum with the
+    exact erfc (the reference's Test8 restates the same three terms).  The
+    A&S erfc of the real-space term is within 1.5e-7 of the exact one, which
+    bounds the difference."""
+    L, rc, tol = 20.0, 9.0, 1e-5
+    s = _two_site_box(5.0, L=L, charge_style="ewald", q=(1.0, -1.0), rc=rc)
+    s.vdw_style = "none"
+    e = oracle_mod.OracleBox(s.parsed()).total_energy()
+    alpha = math.sqrt(-math.log(tol)) / rc
+    cf = S.CHARGE_FACTOR
+    real = cf * (1.0 * -1.0) / 5.0 * math.erfc(alpha * 5.0)
+    hcut = 2.0 * -math.log(tol) / rc
+    nmax = int(hcut * L / (2 * math.pi)) + 1
+    n = np.arange(-nmax, nmax + 1)
+    nx, ny, nz = np.meshgrid(n, n, n, indexing="ij")
+    k = 2 * math.pi / L * np.stack([nx, ny, nz], -1).reshape(-1, 3)
+    k2 = (k * k).sum(1)
+    sel = (k2 > 0) & (k2 < hcut * hcut)
+    k, k2 = k[sel], k2[sel]
+    pos = np.array([[0.0, 0, 0], [5.0, 0, 0]])
+    q = np.array([1.0, -1.0])
+    sk = (q[None, :] * np.exp(1j * k @ pos.T)).sum(1)
+    recip = cf * (2 * math.pi / L ** 3) * (np.exp(-k2 / (4 * alpha * alpha)) / k2 * np.abs(sk) ** 2).sum()
+    self_e = -cf * alpha / math.sqrt(math.pi) * (q * q).sum()
+    assert abs(e["inter_q"] - real) < 2e-7 * cf / 5.0
+    assert abs(e["reciprocal"] - recip) < 1e-6 * abs(recip)
+    assert abs(e["self"] - self_e) < 1e-9 * abs(self_e)
+
+
+def test_translation_invariance(oracle_mod):
+    """A rigid shift of every site leaves every energy term unchanged."""
+    s = config_by_name("sty_cut_ewald").parsed()
+    e0 = oracle_mod.OracleBox(s).total_energy()
+    s2 = S.System(**{**s.__dict__})
+    s2.coords = [c + np.array([1.234, -0.77, 3.3]) for c in s.coords]
+    e1 = oracle_mod.OracleBox(s2).total_energy()
+    for k in ("inter_vdw", "inter_q", "reciprocal", "self", "total"):
+        assert rel_err(e1[k], e0[k]) < 1e-9, k
