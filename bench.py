@@ -1,0 +1,241 @@
+#!/usr/bin/env python3
+"""Benchmark of the hot path on one MI355X.
+
+Workload (the Widom configuration): test insertions of a 3-site CO2 probe
+(charged, Ewald) into a frame of 512 SPC/E-like waters (24.83 A box, LJ 9 A
+cut_tail, Ewald 9 A / 1e-5, 2354 k-vectors).  One step = one batch of 1e6
+insertions, each a full trial energy: the inter energy at the full cutoff and
+the CBMC cutoff, the reciprocal-space change and widom_var = exp(-beta dE).
+
+value = trial energies/s of the batched mixed-precision path (FP32 pair
+terms, FP64 accumulation and FP64 reciprocal sums), inputs resident in HBM.
+Reported beside it: the batched FP64 path, the plain FP64 path that scores
+one insertion per launch, the mixed-vs-FP64 error of dE, NVT moves/s of the
+512-water chain, the roofline of the Widom kernel and the CPU oracle.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_INS = 1_000_000
+FP32_PEAK = 157.3e12
+FP64_PEAK = 78.6e12
+
+
+def widom_frame():
+    from cassandra_amd import systems as S
+    frame = S.config_spce512()
+    return S.with_probe(frame, S.species_co2()).parsed()
+
+
+def algorithmic_flops(system, box_xyz, sp_index, n_sample, seed, engine_mod, eng, box):
+    """FP32 and FP64 flops per insertion from the per-unit weights of the
+    design (pair term: min-image + r^2 = 14, LJ = 12, Ewald real = 34 + exp(4);
+    k term per probe site per k = 12 (phase recurrence) + 6 per k), counting
+    the pairs inside the cutoff for a sample of insertions."""
+    from cassandra_amd import systems as S
+    probe = system.species[sp_index]
+    rng = np.random.default_rng(seed)
+    L = system.box
+    frame = np.concatenate([c.reshape(-1, 3) for c in system.coords if c.size])
+    q = np.concatenate([[a.charge for a in sp.atoms] * n for sp, n in zip(system.species, system.nmols)])
+    lj = np.concatenate([[a.eps_K > 0 for a in sp.atoms] * n for sp, n in zip(system.species, system.nmols)])
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rc2 = system.rcut_vdw ** 2
+    n_lj = n_q = 0
+    for _ in range(n_sample):
+        c = (rng.random(3) - 0.5) * L
+        xyz = ref @ S.random_rotation(rng).T + c
+        for a, at in enumerate(probe.atoms):
+            d = frame - xyz[a]
+            d -= L * np.round(d / L)
+            inr = (d * d).sum(1) <= rc2
+            if at.eps_K > 0:
+                n_lj += int((inr & lj).sum())
+            if at.charge != 0:
+                n_q += int((inr & (q != 0)).sum())
+    n_lj /= n_sample
+    n_q /= n_sample
+    K = box.ewald_setup()
+    f32 = 14 * (n_lj + n_q) / 2 + 12 * n_lj + 38 * n_q   # min-image shared by the two terms of a pair
+    f64 = 12 * probe.natoms * K + 6 * K
+    return f32, f64, n_lj, n_q, K
+
+
+def time_widom(box, sp, n, seed, steps, warmup):
+    for w in range(warmup):
+        box.widom_batch(sp, n, seed + 1000 + w)
+    ks = []
+    t0 = time.perf_counter()
+    for s in range(steps):
+        box.widom_batch(sp, n, seed + s)
+        ks.append(box.eng.last_kernel_ms())
+    dt = time.perf_counter() - t0
+    return dt, float(np.mean(ks))
+
+
+def plain_fp64_per_launch(system, sp, n, seed):
+    """The plain FP64 path: one trial set (one insertion) per launch --
+    cass_trial_energies + cass_ewald_widom for each placement."""
+    from cassandra_amd import engine as E
+    from cassandra_amd import systems as S
+    eng = E.Engine(system, precision=E.FP64)
+    box = eng.add_box(system, max_mols=sum(system.nmols) + 1)
+    box.ewald_setup()
+    box.compute_system_total_energy()
+    probe = system.species[sp]
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rng = np.random.default_rng(seed)
+    place = []
+    for _ in range(n):
+        c = (rng.random(3) - 0.5) * system.box
+        place.append(ref @ S.random_rotation(rng).T + c)
+    box.trial_energies(sp, place[0][None])
+    t0 = time.perf_counter()
+    for p in place:
+        nrg, ov = box.trial_energies(sp, p[None])
+        if not ov[0]:
+            box.ewald_widom(sp, p[None])
+    dt = time.perf_counter() - t0
+    eng.close()
+    return n / dt
+
+
+def cpu_baseline(system, sp, seconds=10.0, seed=11):
+    """The CPU oracle (oracle/liboracle.so, one core) scoring Widom placements
+    of the same workload: a bounded sample sized to about `seconds`."""
+    from oracle import oracle as O
+    from cassandra_amd import systems as S
+    box = O.OracleBox(system)
+    box.total_energy()
+    probe = system.species[sp]
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rng = np.random.default_rng(seed)
+
+    def batch(k):
+        return np.stack([ref @ S.random_rotation(rng).T + (rng.random(3) - 0.5) * system.box for _ in range(k)])
+
+    t0 = time.perf_counter()
+    box.widom_energies(sp, batch(50))
+    t1 = time.perf_counter() - t0
+    n = int(max(50, min(200000, seconds / max(t1 / 50, 1e-6))))
+    pl = batch(n)
+    t0 = time.perf_counter()
+    box.widom_energies(sp, pl)
+    dt = time.perf_counter() - t0
+    return n / dt, n, dt
+
+
+def nvt_moves_per_s(nsteps=2000):
+    from cassandra_amd import engine as E
+    from cassandra_amd import systems as S
+    system = S.config_spce512().parsed()
+    eng = E.Engine(system, precision=E.FP64)
+    box = eng.add_box(system)
+    box.ewald_setup()
+    e0 = box.compute_system_total_energy()
+    r = E.rng(*system.seed)
+    box.nvt_run(r, 50, e0, record=False)
+    e0 = box.compute_system_total_energy()
+    t0 = time.perf_counter()
+    box.nvt_run(r, nsteps, e0, record=False)
+    dt = time.perf_counter() - t0
+    eng.close()
+    return nsteps / dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--insertions", type=int, default=N_INS)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="skip the per-launch and NVT side measurements")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported here "
+              "(insertion ranges would shard across GPUs with one all-reduce of the sums).")
+        sys.exit(2)
+    from cassandra_amd import engine as E
+
+    system = widom_frame()
+    sp = len(system.species) - 1
+    eng = E.Engine(system, precision=E.MIXED)
+    box = eng.add_box(system, max_mols=sum(system.nmols) + 1)
+    box.ewald_setup()
+    box.compute_system_total_energy()
+    probe = system.species[sp]
+    box.widom_set_template(sp, probe.ref_coords)
+    n = a.insertions
+
+    # batched mixed precision: the measured path
+    eng.set_precision(E.MIXED)
+    dt_mixed, k_mixed = time_widom(box, sp, n, 1234, a.steps, a.warmup)
+    value = n * a.steps / dt_mixed
+    # batched FP64
+    eng.set_precision(E.FP64)
+    dt64, k64 = time_widom(box, sp, n, 1234, max(1, a.steps // 2), 1)
+    fp64_rate = n * max(1, a.steps // 2) / dt64
+    # accuracy of the mixed path against FP64 on the same insertions
+    m = 200000
+    eng.set_precision(E.FP64)
+    r64, w64 = box.widom_batch(sp, m, 777, keep=True)
+    eng.set_precision(E.MIXED)
+    r32, w32 = box.widom_batch(sp, m, 777, keep=True)
+    ok = (w64 > 0) & (w32 > 0)
+    beta = system.beta if hasattr(system, "beta") else 1.0
+    de_err = np.abs(np.log(w32[ok]) - np.log(w64[ok])) / beta           # |dE32 - dE64|
+    de64 = np.abs(np.log(w64[ok])) / beta
+    rel_de = de_err / np.maximum(de64, 100.0)
+    mu_rel = abs(r32.widom_sum - r64.widom_sum) / abs(r64.widom_sum)
+
+    f32, f64, n_lj, n_q, K = algorithmic_flops(system, None, sp, 200, 5, E, eng, box)
+    t_ins = k_mixed * 1e-3 / n
+    achieved = (f32 + f64) / t_ins
+    t_ideal = f32 / FP32_PEAK + f64 / FP64_PEAK
+    peak = (f32 + f64) / t_ideal
+    roofline = {"bound": "mfma", "compute": "valu (no MFMA: gather/ALU sums)", "achieved": achieved / 1e12,
+                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                "kernel": "k_widom<float>", "kernel_ms": k_mixed,
+                "flops_per_insertion": {"fp32": f32, "fp64": f64}, "pairs_in_cutoff": {"lj": n_lj, "qq": n_q},
+                "kvectors": K}
+
+    extra = {}
+    if not a.quick:
+        extra["plain_fp64_per_launch_trials_per_s"] = plain_fp64_per_launch(system, sp, 400, 3)
+        extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(1000)
+    cpu = None
+    if not a.no_cpu:
+        rate, ns, dt = cpu_baseline(system, sp, seconds=8.0)
+        cpu = {"value": rate, "unit": "trial energies/s", "cores": 1, "kind": "port",
+               "sample": f"{ns} CO2 insertions into the 512-water frame ({dt:.1f} s, oracle/liboracle.so, "
+                         f"FP64, one thread on {platform.processor() or platform.machine()})"}
+    out = {
+        "metric": "trial energies/s & MC moves/s, batched mixed-prec vs plain FP64 GPU, ΔE<1e-6 rel",
+        "value": value, "unit": "trial energies/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": dt_mixed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32-pair/fp64-accumulate", "data": "synthetic",
+        "config": {"workload": "Widom CO2 -> 512 SPC/E water frame, 1e6 insertions per step",
+                   "insertions_per_step": n, "kvectors": K, "frame_sites": int(system.nsites())},
+        "batched_fp64_trials_per_s": fp64_rate,
+        "speedup_mixed_vs_batched_fp64": value / fp64_rate,
+        "mixed_dE_rel_err": {"max": float(rel_de.max()), "median": float(np.median(rel_de)),
+                             "widom_sum_rel": float(mu_rel)},
+        "roofline": roofline, "cpu_baseline": cpu, **extra,
+    }
+    if "plain_fp64_per_launch_trials_per_s" in extra:
+        out["speedup_vs_plain_fp64_per_launch"] = value / extra["plain_fp64_per_launch_trials_per_s"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -145,6 +145,12 @@ static EwaldDev ewald_view(const cass_box_state &b, const int *row_k0) {
   return w;
 }
 
+BoxDev cass_box_view(const cass_engine *e, const cass_box_state &b) { return box_view(e, b); }
+TopoDev cass_topo_view(const cass_engine *e) { return topo_view(e); }
+EwaldDev cass_ewald_view(const cass_box_state &b) {
+  return ewald_view(b, b.rows ? (const int *)(b.rows + b.nrows) : nullptr);
+}
+
 static bool species_charged(const cass_engine *e, int s) {
   for (int a = 0; a < e->sp_natoms[s]; ++a)
     if (e->atom_charge[e->sp_atom_off[s] + a] != 0.0) return true;
@@ -241,6 +247,29 @@ extern "C" int cass_engine_destroy(cass_engine *e) {
   (void)hipEventDestroy(e->ev1);
   (void)hipStreamDestroy(e->stream);
   delete e;
+  return CASS_OK;
+}
+
+extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const double *sig,
+                                       const double *atom_charge) {
+  if (!e) return CASS_ERR_ARG;
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  const size_t nt2 = (size_t)e->ntypes * e->ntypes;
+  if (eps) {
+    e->eps.assign(eps, eps + nt2);
+    HIP_TRY(e, upload(e, e->d_eps, e->eps.data(), nt2));
+  }
+  if (sig) {
+    e->sig.assign(sig, sig + nt2);
+    HIP_TRY(e, upload(e, e->d_sig, e->sig.data(), nt2));
+  }
+  if (atom_charge) {
+    e->atom_charge.assign(atom_charge, atom_charge + e->natoms_total);
+    HIP_TRY(e, upload(e, e->d_atom_charge, e->atom_charge.data(), e->natoms_total));
+  }
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  for (auto &b : e->boxes) b.ewald_ready = b.ewald_ready && !atom_charge;
   return CASS_OK;
 }
 

@@ -49,81 +49,6 @@ __device__ inline void block_reduce(double (&v)[NV], int &flag) {
   __syncthreads();
 }
 
-// ------------------------------------------------------------- geometry
-__device__ inline void min_image(const BoxDev &b, double &dx, double &dy, double &dz) {
-  // minimum_image_separation.f90:35-112: cubic boxes shift by one box length
-  // at most; other boxes round fractional coordinates.
-  if (b.cubic) {
-    if (dx > b.hL[0]) dx -= b.L[0]; else if (dx < -b.hL[0]) dx += b.L[0];
-    if (dy > b.hL[1]) dy -= b.L[1]; else if (dy < -b.hL[1]) dy += b.L[1];
-    if (dz > b.hL[2]) dz -= b.L[2]; else if (dz < -b.hL[2]) dz += b.L[2];
-  } else {
-    double sx = b.Li[0] * dx + b.Li[3] * dy + b.Li[6] * dz;
-    double sy = b.Li[1] * dx + b.Li[4] * dy + b.Li[7] * dz;
-    double sz = b.Li[2] * dx + b.Li[5] * dy + b.Li[8] * dz;
-    sx -= rint_away(sx);
-    sy -= rint_away(sy);
-    sz -= rint_away(sz);
-    dx = b.Lc[0] * sx + b.Lc[3] * sy + b.Lc[6] * sz;
-    dy = b.Lc[1] * sx + b.Lc[4] * sy + b.Lc[7] * sz;
-    dz = b.Lc[2] * sx + b.Lc[5] * sy + b.Lc[8] * sz;
-  }
-}
-
-// Check_MoleculePair_Cutoff (energy_routines.f90:3027-3088).
-__device__ inline bool molpair_in_range(const BoxDev &b, bool cbmc, double4 ci, double4 cj) {
-  if (b.vdw_minimum) return true;
-  double dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
-  min_image(b, dx, dy, dz);
-  double rcom = sqrt(dx * dx + dy * dy + dz * dz);
-  double rint = (cbmc ? b.rcut_cbmc : b.rcut_max) + ci.w + cj.w;
-  return !(rcom > rint);
-}
-
-// One probe molecule (sites px/py/pz with species-atom ids pspa) against
-// stored molecule j: Compute_MoleculePair_Energy (energy_routines.f90:1589-1730).
-// Accumulates vdW and q-q in FP64; returns true on overlap r^2 < rcut_low^2.
-// With both_cutoffs the same pass also accumulates the CBMC-cutoff sum.
-template <typename T>
-__device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const PairConst<double> &cd,
-                                      const PairConst<T> &ct, bool cbmc, bool coul_cbmc, int ni,
-                                      const double *px, const double *py, const double *pz,
-                                      const int *pspa, int j, double &ev, double &eq, double *ecbmc) {
-  const int sj = b.species[j], fj = b.first[j];
-  const int nj = tp.sp_natoms[sj], oj = tp.sp_atom_off[sj];
-  for (int ia = 0; ia < ni; ++ia) {
-    const int ti = tp.atom_type[pspa[ia]];
-    const double qi = tp.atom_charge[pspa[ia]];
-    for (int ja = 0; ja < nj; ++ja) {
-      double dx = px[ia] - b.x[fj + ja], dy = py[ia] - b.y[fj + ja], dz = pz[ia] - b.z[fj + ja];
-      min_image(b, dx, dy, dz);
-      const double r2d = dx * dx + dy * dy + dz * dz;
-      if (r2d < cd.rc_low2) return true;
-      bool gv, gq;
-      pair_cutoff(cd, cbmc, r2d, gv, gq);
-      if (cbmc && !coul_cbmc) gq = false;
-      if (!(gv || gq) && !ecbmc) continue;
-      const int tj = tp.atom_type[oj + ja];
-      const T r2 = (T)r2d;
-      T e_v = (T)0, e_q = (T)0;
-      if (ti >= 0 && tj >= 0 && (gv || (ecbmc && r2d <= cd.rc_cbmc2))) {
-        const int k = ti * tp.ntypes + tj;
-        e_v = vdw_inter<T>(ct, r2, (T)tp.eps[k], (T)tp.sig[k], (T)tp.mn[k], (T)tp.mm[k]);
-      }
-      const double qj = tp.atom_charge[oj + ja];
-      const bool any_q = ct.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
-      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<T>(ct, r2, (T)(qi * qj));
-      if (gv) ev += (double)e_v;
-      if (gq) eq += (double)e_q;
-      if (ecbmc && r2d <= cd.rc_cbmc2) {
-        bool cv = ti >= 0 && tj >= 0 && cd.vdw_style != CASS_VDW_NONE;
-        *ecbmc += (cv ? (double)e_v : 0.0) + ((any_q && coul_cbmc) ? (double)e_q : 0.0);
-      }
-    }
-  }
-  return false;
-}
-
 // ------------------------------------------------------ trial energies
 // grid (nblk, ntrial): trial t against the box's molecules
 // [blockIdx.x*blockDim.x, +blockDim.x).  Partials per (trial, block):

@@ -78,11 +78,17 @@ def load_library(path=LIB_PATH):
         "cass_nvt_run": (i32, [vp, i32, P(abi.CassRng), P(abi.CassNvtOptions), i64, P(abi.CassEnergy),
                                P(abi.CassNvtRecord)]),
     }
-    optional = {
+    sig.update({
         "cass_trial_energies_multi": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i64), P(i64), P(d), i32, P(i32),
                                             P(d), P(i32)]),
         "cass_adjoint": (i32, [vp, i32, P(d), P(d), P(d), P(d), P(d)]),
-    }
+        "cass_set_nonbond_params": (i32, [vp, P(d), P(d), P(d)]),
+        "cass_trial_plan_create": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i32), i32, P(vp)]),
+        "cass_trial_plan_run": (i32, [vp, P(d), i32, P(d), P(i32)]),
+        "cass_trial_plan_autotune": (i32, [vp, P(d), P(i32)]),
+        "cass_trial_plan_destroy": (i32, [vp]),
+    })
+    optional = {}
     for name, (res, args) in list(sig.items()) + list(optional.items()):
         if not hasattr(lib, name):
             if name in optional:
@@ -147,6 +153,15 @@ class Engine:
     def set_precision(self, prec):
         self._chk(self.lib.cass_set_precision(self.h, prec), "cass_set_precision")
         self.precision = prec
+
+    def set_nonbond_params(self, eps=None, sig=None, charges=None):
+        """cass_set_nonbond_params: full ntypes x ntypes tables (internal units)
+        and/or per-atom charges; None keeps the current one."""
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (eps, sig, charges)]
+        self._chk(self.lib.cass_set_nonbond_params(self.h, *[_p(a) for a in arrs]), "cass_set_nonbond_params")
+
+    def trial_plan(self, boxes, species, ntrial, exclude=None, cbmc=True):
+        return TrialPlan(self, boxes, species, ntrial, exclude, cbmc)
 
     def add_box(self, system: S.System, max_mols=None, fold=True):
         """cass_box_create + cass_box_upload of a System (coordinates folded
@@ -335,6 +350,64 @@ class Box:
 
     def params_prob_translate(self):
         return self.system.prob_translate
+
+    # --------------------------------------------------------------- adjoint
+    def adjoint(self):
+        """cass_adjoint: (E_nonbonded, dE/d eps_t, dE/d sigma_t, dE/d q_kind, forces[nsites,3])."""
+        nt = self.eng.topo.ntypes
+        nk = len(self.eng.topo.atom_charge)
+        e = C.c_double()
+        de = np.zeros(nt)
+        ds = np.zeros(nt)
+        dq = np.zeros(nk)
+        f = np.zeros(3 * self.nsites)
+        self.eng._chk(self.lib.cass_adjoint(self.eng.h, self.id, C.byref(e), _p(de), _p(ds), _p(dq), _p(f)),
+                      "cass_adjoint")
+        return e.value, de, ds, dq, f.reshape(-1, 3)
+
+
+class TrialPlan:
+    """cass_trial_plan: trial sets of several boxes in one launch, optionally
+    replayed from a captured hipGraph."""
+
+    def __init__(self, eng: Engine, boxes, species, ntrial, exclude=None, cbmc=True):
+        self.eng = eng
+        n = len(boxes)
+        self.boxes = np.ascontiguousarray([b.id if isinstance(b, Box) else b for b in boxes], dtype=np.int32)
+        self.species = np.ascontiguousarray(species, dtype=np.int32)
+        self.ntrial = np.ascontiguousarray(ntrial, dtype=np.int32)
+        self.exclude = np.ascontiguousarray(exclude if exclude is not None else [-1] * n, dtype=np.int32)
+        h = C.c_void_p()
+        eng._chk(eng.lib.cass_trial_plan_create(eng.h, n, _p(self.boxes, C.c_int32), _p(self.species, C.c_int32),
+                                                _p(self.ntrial, C.c_int32), _p(self.exclude, C.c_int32), int(cbmc),
+                                                C.byref(h)), "cass_trial_plan_create")
+        self.h = h
+        self.total = int(self.ntrial.sum())
+
+    def run(self, xyz, graph=True):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        nrg = np.zeros(self.total)
+        ov = np.zeros(self.total, dtype=np.int32)
+        self.eng._chk(self.eng.lib.cass_trial_plan_run(self.h, _p(x), int(graph), _p(nrg), _p(ov, C.c_int32)),
+                      "cass_trial_plan_run")
+        return nrg, ov.astype(bool)
+
+    def autotune(self, xyz):
+        x = np.ascontiguousarray(xyz, dtype=np.float64)
+        blk = C.c_int32()
+        self.eng._chk(self.eng.lib.cass_trial_plan_autotune(self.h, _p(x), C.byref(blk)), "cass_trial_plan_autotune")
+        return blk.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.eng.lib.cass_trial_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def rng(i1, i3) -> abi.CassRng:

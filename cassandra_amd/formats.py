@@ -122,7 +122,9 @@ def write_xyz(system: System, path: str):
         for m in range(xyz.shape[0]):
             for ia, a in enumerate(sp.atoms):
                 x, y, z = xyz[m, ia]
-                rows.append(f"{a.element} {_r(x)} {_r(y)} {_r(z)}")
+                # frames are read with list-directed READ (read_write_checkpoint.f90:442),
+                # which rounds correctly: repr round-trips every double
+                rows.append(f"{a.element} {float(x)!r} {float(y)!r} {float(z)!r}")
     with open(path, "w") as f:
         f.write(f"{len(rows)}\n\n" + "\n".join(rows) + "\n")
 

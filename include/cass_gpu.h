@@ -129,6 +129,14 @@ int cass_engine_create(int32_t device, const cass_topology *topo, int32_t max_bo
                        cass_engine **out);
 int cass_engine_destroy(cass_engine *eng);
 int cass_set_precision(cass_engine *eng, int32_t prec);   /* CASS_PREC_* */
+
+/* Replaces the nonbond tables (vdw_param1/2_table, [ntypes*ntypes]) and/or
+ * the per-atom charges ([sum natoms]) of the topology; NULL keeps a table.
+ * The Fortran host would call it after Create_Nonbond_Table
+ * (create_nonbond_table.f90:10) reruns; force-field fitting calls it between
+ * adjoint evaluations.  Ewald sums must be recomputed after a charge change. */
+int cass_set_nonbond_params(cass_engine *eng, const double *eps, const double *sig,
+                            const double *atom_charge);
 const char *cass_last_error(cass_engine *eng);
 /* Device time of the last batched kernel (Widom / trial-set), from HIP events
  * recorded on the engine's stream around the launch.  Instrumentation; no
@@ -200,6 +208,24 @@ int cass_trial_energies_multi(cass_engine *eng, int32_t nsets, const int32_t *bo
                               const int64_t *xyz_offset, const int64_t *out_offset,
                               const double *trial_xyz, int32_t cbmc_flag,
                               const int32_t *exclude_mol, double *nrg, int32_t *overlap);
+
+/* A reusable trial plan for cass_trial_energies_multi-style launches: the
+ * sets (box, species, trial count, excluded molecule) are fixed at creation;
+ * every run takes fresh coordinates (sets back to back, trial-major) and
+ * returns E_vdw + E_qq and the overlap flag per trial.  With use_graph the
+ * run (copy in, trial kernel, finalize, copy out) is captured once in a
+ * hipGraph and replayed.  cass_trial_plan_autotune times the molecule-block
+ * sizes 64/128/256 on the plan's boxes and keeps the fastest.  The sets
+ * replace the per-box kappa loops of Build_Molecule / Fragment_Placement
+ * (fragment_growth.f90:383-579, 1926-2058) run for many boxes at once. */
+typedef struct cass_trial_plan cass_trial_plan;
+int cass_trial_plan_create(cass_engine *eng, int32_t nsets, const int32_t *box_of_set,
+                           const int32_t *species_of_set, const int32_t *ntrial,
+                           const int32_t *exclude_mol, int32_t cbmc_flag, cass_trial_plan **plan);
+int cass_trial_plan_run(cass_trial_plan *plan, const double *trial_xyz, int32_t use_graph,
+                        double *nrg, int32_t *overlap);
+int cass_trial_plan_autotune(cass_trial_plan *plan, const double *trial_xyz, int32_t *block);
+int cass_trial_plan_destroy(cass_trial_plan *plan);
 
 /* Compute_System_Total_Energy (energy_routines.f90:2648-2953) with intra
  * terms restricted to the nonbonded part (bonded terms stay on the host):
