@@ -274,7 +274,7 @@ extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const 
 }
 
 extern "C" int cass_set_precision(cass_engine *e, int32_t prec) {
-  if (!e || (prec != CASS_PREC_FP64 && prec != CASS_PREC_MIXED)) return CASS_ERR_ARG;
+  if (!e || (prec != CASS_PREC_FP64 && prec != CASS_PREC_MIXED && prec != CASS_PREC_FP32)) return CASS_ERR_ARG;
   e->prec = prec;
   return CASS_OK;
 }
@@ -459,14 +459,12 @@ static int launch_trials(cass_engine *e, cass_box_state &b, int species, int ntr
   TopoDev tv = topo_view(e);
   PairConst<double> cd = make_pair_const<double>(b.p, e->ntypes);
   dim3 grid(nblk, ntrial);
-  if (e->prec == CASS_PREC_MIXED) {
-    PairConst<float> cf = make_pair_const<float>(b.p, e->ntypes);
-    hipLaunchKernelGGL(k_trials<float>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, species, ni, d_txyz,
-                       exclude, cbmc, part, part_ov);
-  } else {
-    hipLaunchKernelGGL(k_trials<double>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cd, species, ni, d_txyz,
-                       exclude, cbmc, part, part_ov);
-  }
+  PairConst<float> cf = make_pair_const<float>(b.p, e->ntypes);
+  auto kern = e->prec == CASS_PREC_MIXED  ? k_trials<PrecMixed>
+              : e->prec == CASS_PREC_FP32 ? k_trials<PrecFP32>
+                                          : k_trials<PrecFP64>;
+  hipLaunchKernelGGL(kern, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, species, ni, d_txyz, exclude, cbmc, part,
+                     part_ov);
   HIP_TRY(e, hipGetLastError());
   hipLaunchKernelGGL(k_finalize_trials, dim3((ntrial + 255) / 256), dim3(256), 0, e->stream, part, part_ov,
                      nblk, ntrial, dev_out, dev_ov);
@@ -600,12 +598,11 @@ extern "C" int cass_total_energy(cass_engine *e, int32_t box, int32_t intra_flag
     TopoDev tv = topo_view(e);
     PairConst<double> cd = make_pair_const<double>(b->p, e->ntypes);
     dim3 grid(nblk, b->nmols);
-    if (e->prec == CASS_PREC_MIXED) {
-      PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
-      hipLaunchKernelGGL(k_total_pairs<float>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, part, part_ov);
-    } else {
-      hipLaunchKernelGGL(k_total_pairs<double>, grid, dim3(blk), 0, e->stream, bv, tv, cd, cd, part, part_ov);
-    }
+    PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
+    auto kern = e->prec == CASS_PREC_MIXED  ? k_total_pairs<PrecMixed>
+                : e->prec == CASS_PREC_FP32 ? k_total_pairs<PrecFP32>
+                                            : k_total_pairs<PrecFP64>;
+    hipLaunchKernelGGL(kern, grid, dim3(blk), 0, e->stream, bv, tv, cd, cf, part, part_ov);
     HIP_TRY(e, hipGetLastError());
     std::vector<double> hp(2 * np);
     std::vector<int> ho(np);
@@ -940,14 +937,12 @@ static int run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins
   }
   PairConst<double> cd = make_pair_const<double>(b->p, e->ntypes);
   HIP_TRY(e, hipEventRecord(e->ev0, e->stream));
-  if (e->prec == CASS_PREC_MIXED) {
-    PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
-    hipLaunchKernelGGL(k_widom<float>, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cf,
-                       part, part_ov, d_var, d_eout);
-  } else {
-    hipLaunchKernelGGL(k_widom<double>, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cd,
-                       part, part_ov, d_var, d_eout);
-  }
+  PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
+  auto kern = e->prec == CASS_PREC_MIXED  ? k_widom<PrecMixed>
+              : e->prec == CASS_PREC_FP32 ? k_widom<PrecFP32>
+                                          : k_widom<PrecFP64>;
+  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cf, part, part_ov, d_var,
+                     d_eout);
   HIP_TRY(e, hipGetLastError());
   HIP_TRY(e, hipEventRecord(e->ev1, e->stream));
   hipLaunchKernelGGL(k_sum_widom, dim3(1), dim3(256), 0, e->stream, part, part_ov, nblk, d_sum, d_nov);

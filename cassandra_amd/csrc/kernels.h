@@ -87,15 +87,51 @@ __device__ inline bool molpair_in_range(const BoxDev &b, bool cbmc, double4 ci, 
   return !(rcom > rint);
 }
 
+// Arithmetic of the two pair terms per precision mode: V for vdW, Q for the
+// real-space q-q term.  Mixed keeps LJ in FP64 (s^12 - s^6 cancels near
+// r = sigma, where FP32 loses ~1e-5 of the pair energy) and moves the
+// transcendental-heavy Coulomb term (rcp, sqrt, exp of the A&S erfc) to FP32.
+struct PrecFP64 {
+  using V = double;
+  using Q = double;
+};
+struct PrecMixed {
+  using V = double;
+  using Q = float;
+};
+struct PrecFP32 {
+  using V = float;
+  using Q = float;
+};
+
+template <typename T>
+__host__ __device__ inline const PairConst<T> &pconst(const PairConst<double> &cd, const PairConst<float> &cf);
+template <>
+__host__ __device__ inline const PairConst<double> &pconst<double>(const PairConst<double> &cd,
+                                                                   const PairConst<float> &) {
+  return cd;
+}
+template <>
+__host__ __device__ inline const PairConst<float> &pconst<float>(const PairConst<double> &,
+                                                                 const PairConst<float> &cf) {
+  return cf;
+}
+
 // One probe molecule (sites px/py/pz with species-atom ids pspa) against
 // stored molecule j: Compute_MoleculePair_Energy (energy_routines.f90:1589-1730).
 // Accumulates vdW and q-q in FP64; returns true on overlap r^2 < rcut_low^2.
-// With both_cutoffs the same pass also accumulates the CBMC-cutoff sum.
-template <typename T>
+// Cutoff and overlap tests use the FP64 r^2 in every mode, so all modes see
+// the same pair set.  With ecbmc the same pass also accumulates the
+// CBMC-cutoff sum.
+template <typename P>
 __device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const PairConst<double> &cd,
-                                      const PairConst<T> &ct, bool cbmc, bool coul_cbmc, int ni,
+                                      const PairConst<float> &cf, bool cbmc, bool coul_cbmc, int ni,
                                       const double *px, const double *py, const double *pz,
                                       const int *pspa, int j, double &ev, double &eq, double *ecbmc) {
+  using TV = typename P::V;
+  using TQ = typename P::Q;
+  const PairConst<TV> &cv = pconst<TV>(cd, cf);
+  const PairConst<TQ> &cq = pconst<TQ>(cd, cf);
   const int sj = b.species[j], fj = b.first[j];
   const int nj = tp.sp_natoms[sj], oj = tp.sp_atom_off[sj];
   for (int ia = 0; ia < ni; ++ia) {
@@ -111,15 +147,15 @@ __device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const 
       if (cbmc && !coul_cbmc) gq = false;
       if (!(gv || gq) && !ecbmc) continue;
       const int tj = tp.atom_type[oj + ja];
-      const T r2 = (T)r2d;
-      T e_v = (T)0, e_q = (T)0;
+      TV e_v = (TV)0;
+      TQ e_q = (TQ)0;
       if (ti >= 0 && tj >= 0 && (gv || (ecbmc && r2d <= cd.rc_cbmc2))) {
         const int k = ti * tp.ntypes + tj;
-        e_v = vdw_inter<T>(ct, r2, (T)tp.eps[k], (T)tp.sig[k], (T)tp.mn[k], (T)tp.mm[k]);
+        e_v = vdw_inter<TV>(cv, (TV)r2d, (TV)tp.eps[k], (TV)tp.sig[k], (TV)tp.mn[k], (TV)tp.mm[k]);
       }
       const double qj = tp.atom_charge[oj + ja];
-      const bool any_q = ct.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
-      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<T>(ct, r2, (T)(qi * qj));
+      const bool any_q = cd.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
+      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<TQ>(cq, (TQ)r2d, (TQ)(qi * qj));
       if (gv) ev += (double)e_v;
       if (gq) eq += (double)e_q;
       if (ecbmc && r2d <= cd.rc_cbmc2) {
@@ -270,14 +306,14 @@ __device__ inline double widom_recip_delta(const EwaldDev &ew, int n, const doub
 }
 
 // kernels (kernels.hip)
-template <typename T>
-__global__ void k_trials(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<T> ct, int species, int ni,
+template <typename P>
+__global__ void k_trials(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<float> cf, int species, int ni,
                          const double *txyz, int exclude, int cbmc, double *part, int *part_ov);
 __global__ void k_finalize_trials(const double *part, const int *part_ov, int nblk, int ntrial, double *out,
                                   int *out_ov);
 __global__ void k_sum(const double *in, size_t n, int stride, int comp, double *out);
-template <typename T>
-__global__ void k_total_pairs(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<T> ct, double *part,
+template <typename P>
+__global__ void k_total_pairs(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<float> cf, double *part,
                               int *part_ov);
 __global__ void k_ewald_mol(BoxDev b, TopoDev tp, EwaldDev ew);
 __global__ void k_ewald_box(BoxDev b, EwaldDev ew, double *part);
@@ -285,9 +321,9 @@ __global__ void k_ewald_move(TopoDev tp, EwaldDev ew, int m, int species, int fl
                              double *part);
 __global__ void k_ewald_commit(EwaldDev ew, int m);
 __global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const double *txyz, double *part);
-template <typename T>
+template <typename P>
 __global__ void k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew, PairConst<double> cd,
-                        PairConst<T> ct, double *part, int64_t *part_ov, double *wvar, double *eout);
+                        PairConst<float> cf, double *part, int64_t *part_ov, double *wvar, double *eout);
 __global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, double *out, int64_t *out_ov);
 
 }  // namespace cass

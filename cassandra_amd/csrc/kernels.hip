@@ -53,9 +53,9 @@ __device__ inline void block_reduce(double (&v)[NV], int &flag) {
 // grid (nblk, ntrial): trial t against the box's molecules
 // [blockIdx.x*blockDim.x, +blockDim.x).  Partials per (trial, block):
 // part[(t*nblk + bx)*2 + {0,1}] = {E_vdw, E_qq}, part_ov[t*nblk + bx].
-template <typename T>
+template <typename P>
 __global__ void __launch_bounds__(kMaxBlock) k_trials(BoxDev b, TopoDev tp, PairConst<double> cd,
-                                                      PairConst<T> ct, int species, int ni,
+                                                      PairConst<float> cf, int species, int ni,
                                                       const double *__restrict__ txyz, int exclude,
                                                       int cbmc, double *part, int *part_ov) {
   __shared__ double px[kMaxSites], py[kMaxSites], pz[kMaxSites];
@@ -79,7 +79,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_trials(BoxDev b, TopoDev tp, Pair
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < b.nmols && j != exclude && b.live[j]) {
     if (molpair_in_range(b, cbmc, pcom, b.comd[j]))
-      ov = molpair_energy<T>(b, tp, cd, ct, cbmc, coul_cbmc, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
+      ov = molpair_energy<P>(b, tp, cd, cf, cbmc, coul_cbmc, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
   }
   block_reduce<2>(v, ov);
   if (threadIdx.x == 0) {
@@ -120,9 +120,9 @@ __global__ void k_sum(const double *in, size_t n, int stride, int comp, double *
 // ---------------------------------------------------- total pair energy
 // Compute_System_Total_Energy inter part (energy_routines.f90:2775-2917):
 // grid (nblk, nmols); block (bx, i) covers pairs (i, j) with j > i.
-template <typename T>
+template <typename P>
 __global__ void __launch_bounds__(kMaxBlock) k_total_pairs(BoxDev b, TopoDev tp, PairConst<double> cd,
-                                                           PairConst<T> ct, double *part, int *part_ov) {
+                                                           PairConst<float> cf, double *part, int *part_ov) {
   __shared__ double px[kMaxSites], py[kMaxSites], pz[kMaxSites];
   __shared__ int pspa[kMaxSites];
   const int i = blockIdx.y;
@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_total_pairs(BoxDev b, TopoDev tp,
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (b.live[i] && j > i && j < b.nmols && b.live[j]) {
     if (molpair_in_range(b, false, b.comd[i], b.comd[j]))
-      ov = molpair_energy<T>(b, tp, cd, ct, false, true, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
+      ov = molpair_energy<P>(b, tp, cd, cf, false, true, ni, px, py, pz, pspa, j, v[0], v[1], nullptr);
   }
   block_reduce<2>(v, ov);
   if (threadIdx.x == 0) {
@@ -298,9 +298,9 @@ __global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const
 // and the CBMC cutoff (overlap semantics of Build_Molecule,
 // fragment_growth.f90:519-544); reciprocal delta over the k rows by phase
 // recurrence in FP64; widom_var = exp(-beta (E_inter + dE_recip + e_const)).
-template <typename T>
+template <typename P>
 __global__ void __launch_bounds__(kWidomBlock) k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew,
-                                                       PairConst<double> cd, PairConst<T> ct,
+                                                       PairConst<double> cd, PairConst<float> cf,
                                                        double *part, int64_t *part_ov, double *wvar,
                                                        double *eout) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(kWidomBlock) k_widom(WidomArgs w, BoxDev b, To
       // the full-cutoff prefilter is the wider one; the CBMC sum is taken in
       // the same pass for the pairs within rcut_cbmc.
       if (!molpair_in_range(b, false, pc, cj) && !molpair_in_range(b, true, pc, cj)) continue;
-      over = molpair_energy<T>(b, tp, cd, ct, false, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, &ecb);
+      over = molpair_energy<P>(b, tp, cd, cf, false, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, &ecb);
     }
     double e_inter = ev + eq;
     double de_recip = 0.0;
@@ -399,17 +399,16 @@ __global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, d
 }
 
 // ---------------------------------------------------------- instantiation
-template __global__ void k_trials<double>(BoxDev, TopoDev, PairConst<double>, PairConst<double>, int, int,
-                                          const double *, int, int, double *, int *);
-template __global__ void k_trials<float>(BoxDev, TopoDev, PairConst<double>, PairConst<float>, int, int,
-                                         const double *, int, int, double *, int *);
-template __global__ void k_total_pairs<double>(BoxDev, TopoDev, PairConst<double>, PairConst<double>,
-                                               double *, int *);
-template __global__ void k_total_pairs<float>(BoxDev, TopoDev, PairConst<double>, PairConst<float>,
-                                              double *, int *);
-template __global__ void k_widom<double>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,
-                                         PairConst<double>, double *, int64_t *, double *, double *);
-template __global__ void k_widom<float>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,
-                                        PairConst<float>, double *, int64_t *, double *, double *);
+#define CASS_INSTANTIATE(P)                                                                               \
+  template __global__ void k_trials<P>(BoxDev, TopoDev, PairConst<double>, PairConst<float>, int, int,    \
+                                       const double *, int, int, double *, int *);                        \
+  template __global__ void k_total_pairs<P>(BoxDev, TopoDev, PairConst<double>, PairConst<float>,         \
+                                            double *, int *);                                             \
+  template __global__ void k_widom<P>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,            \
+                                      PairConst<float>, double *, int64_t *, double *, double *);
+CASS_INSTANTIATE(PrecFP64)
+CASS_INSTANTIATE(PrecMixed)
+CASS_INSTANTIATE(PrecFP32)
+#undef CASS_INSTANTIATE
 
 }  // namespace cass

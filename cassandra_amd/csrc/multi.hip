@@ -38,15 +38,8 @@ struct BoxConst {
   PairConst<float> cf;
 };
 
-template <typename T>
-__device__ inline const PairConst<T> &pick(const BoxConst &c);
-template <>
-__device__ inline const PairConst<double> &pick<double>(const BoxConst &c) { return c.cd; }
-template <>
-__device__ inline const PairConst<float> &pick<float>(const BoxConst &c) { return c.cf; }
-
 // One workgroup per item (set, trial, molecule block).
-template <typename T>
+template <typename P>
 __global__ void __launch_bounds__(kMaxBlock) k_trials_multi(const BoxConst *__restrict__ boxes, TopoDev tp,
                                                             const SetDev *__restrict__ sets,
                                                             const int4 *__restrict__ items,
@@ -75,7 +68,7 @@ __global__ void __launch_bounds__(kMaxBlock) k_trials_multi(const BoxConst *__re
   const int j = it.z * blockDim.x + threadIdx.x;
   if (j < bc.b.nmols && j != sd.exclude && bc.b.live[j]) {
     if (molpair_in_range(bc.b, cbmc, pcom, bc.b.comd[j]))
-      ov = molpair_energy<T>(bc.b, tp, bc.cd, pick<T>(bc), cbmc, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, nullptr);
+      ov = molpair_energy<P>(bc.b, tp, bc.cd, bc.cf, cbmc, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, nullptr);
   }
   // block reduction
   __shared__ double sh[2][kMaxBlock / 64];
@@ -217,12 +210,11 @@ static void plan_enqueue(cass_trial_plan *p) {
   (void)hipMemcpyAsync(p->d_boxes, p->h_boxes, sizeof(BoxConst) * p->nboxes, hipMemcpyHostToDevice, e->stream);
   (void)hipMemcpyAsync(p->d_xyz, p->h_xyz, sizeof(double) * p->nxyz, hipMemcpyHostToDevice, e->stream);
   TopoDev tv = cass_topo_view(e);
-  if (e->prec == CASS_PREC_MIXED)
-    hipLaunchKernelGGL(k_trials_multi<float>, dim3(p->nitems), dim3(p->block), 0, e->stream, p->d_boxes, tv,
-                       p->d_sets, p->d_items, p->d_xyz, p->cbmc, p->d_part, p->d_part_ov);
-  else
-    hipLaunchKernelGGL(k_trials_multi<double>, dim3(p->nitems), dim3(p->block), 0, e->stream, p->d_boxes, tv,
-                       p->d_sets, p->d_items, p->d_xyz, p->cbmc, p->d_part, p->d_part_ov);
+  auto kern = e->prec == CASS_PREC_MIXED  ? k_trials_multi<PrecMixed>
+              : e->prec == CASS_PREC_FP32 ? k_trials_multi<PrecFP32>
+                                          : k_trials_multi<PrecFP64>;
+  hipLaunchKernelGGL(kern, dim3(p->nitems), dim3(p->block), 0, e->stream, p->d_boxes, tv, p->d_sets, p->d_items,
+                     p->d_xyz, p->cbmc, p->d_part, p->d_part_ov);
   const int nt = (int)p->ntrial_tot;
   hipLaunchKernelGGL(k_finalize_multi, dim3((nt + 255) / 256), dim3(256), 0, e->stream, p->d_sets, p->d_trials, nt,
                      p->d_part, p->d_part_ov, p->d_out, p->d_out_ov);
@@ -333,6 +325,29 @@ extern "C" int cass_trial_plan_run(cass_trial_plan *p, const double *trial_xyz, 
     nrg[t] = p->h_out[2 * t] + p->h_out[2 * t + 1];
     overlap[t] = p->h_out_ov[t];
   }
+  return CASS_OK;
+}
+
+// New species and/or excluded molecule per set for the next runs; a species
+// change must keep the set's site count (the coordinate layout is fixed).
+// The set table lives in device memory, so a captured graph sees the change.
+extern "C" int cass_trial_plan_update(cass_trial_plan *p, const int32_t *species_of_set,
+                                      const int32_t *exclude_mol) {
+  if (!p) return CASS_ERR_ARG;
+  cass_engine *e = p->e;
+  for (int s = 0; s < p->nsets; ++s) {
+    SetDev &sd = p->sets[s];
+    if (species_of_set) {
+      const int ns = species_of_set[s];
+      if (ns < 0 || ns >= e->nspecies || e->sp_natoms[ns] != e->sp_natoms[sd.species])
+        return cass_fail(e, CASS_ERR_ARG, "species change must keep the site count");
+      sd.species = ns;
+    }
+    if (exclude_mol) sd.exclude = exclude_mol[s];
+  }
+  if (hipSetDevice(e->device) != hipSuccess || hipStreamSynchronize(e->stream) != hipSuccess ||
+      hipMemcpy(p->d_sets, p->sets.data(), sizeof(SetDev) * p->nsets, hipMemcpyHostToDevice) != hipSuccess)
+    return cass_fail(e, CASS_ERR_HIP, "trial plan update");
   return CASS_OK;
 }
 

@@ -9,6 +9,7 @@ library is missing or no GPU is visible, construction raises.
 from __future__ import annotations
 
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -20,7 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcassgpu.so")
 
 FP64 = 0
-MIXED = 1
+MIXED = 1     # FP32 real-space q-q pair terms, FP64 LJ/Mie and sums
+FP32 = 2      # FP32 for both pair terms, FP64 sums
 
 _STATUS = {0: "ok", 1: "bad argument", 2: "HIP error", 3: "out of device memory", 4: "too many k vectors",
            5: "net charge", 6: "call out of order", 7: "no GPU visible", 8: "cutoff exceeds half the box"}
@@ -87,6 +89,7 @@ def load_library(path=LIB_PATH):
         "cass_trial_plan_run": (i32, [vp, P(d), i32, P(d), P(i32)]),
         "cass_trial_plan_autotune": (i32, [vp, P(d), P(i32)]),
         "cass_trial_plan_destroy": (i32, [vp]),
+        "cass_trial_plan_update": (i32, [vp, P(i32), P(i32)]),
     })
     optional = {}
     for name, (res, args) in list(sig.items()) + list(optional.items()):
@@ -128,10 +131,15 @@ class Engine:
             raise CassError(rc, "cass_engine_create (is a GPU visible?)")
         self.h = h
         self.boxes = []
+        self._plans = weakref.WeakSet()
         self.set_precision(precision)
 
     def close(self):
+        """Destroys the engine; plans built on it go first (they hold the
+        engine's stream and graph executables)."""
         if getattr(self, "h", None):
+            for plan in list(getattr(self, "_plans", ())):
+                plan.close()
             self.lib.cass_engine_destroy(self.h)
             self.h = None
 
@@ -161,7 +169,9 @@ class Engine:
         self._chk(self.lib.cass_set_nonbond_params(self.h, *[_p(a) for a in arrs]), "cass_set_nonbond_params")
 
     def trial_plan(self, boxes, species, ntrial, exclude=None, cbmc=True):
-        return TrialPlan(self, boxes, species, ntrial, exclude, cbmc)
+        plan = TrialPlan(self, boxes, species, ntrial, exclude, cbmc)
+        self._plans.add(plan)
+        return plan
 
     def add_box(self, system: S.System, max_mols=None, fold=True):
         """cass_box_create + cass_box_upload of a System (coordinates folded
@@ -392,6 +402,12 @@ class TrialPlan:
                       "cass_trial_plan_run")
         return nrg, ov.astype(bool)
 
+    def update(self, species=None, exclude=None):
+        s = None if species is None else np.ascontiguousarray(species, dtype=np.int32)
+        x = None if exclude is None else np.ascontiguousarray(exclude, dtype=np.int32)
+        self.eng._chk(self.eng.lib.cass_trial_plan_update(self.h, _p(s, C.c_int32), _p(x, C.c_int32)),
+                      "cass_trial_plan_update")
+
     def autotune(self, xyz):
         x = np.ascontiguousarray(xyz, dtype=np.float64)
         blk = C.c_int32()
@@ -400,7 +416,8 @@ class TrialPlan:
 
     def close(self):
         if getattr(self, "h", None):
-            self.eng.lib.cass_trial_plan_destroy(self.h)
+            if getattr(self.eng, "h", None):
+                self.eng.lib.cass_trial_plan_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -346,6 +346,46 @@ def com_of(sp: Species, xyz):
     return (m[:, None] * xyz).sum(0) / m.sum()
 
 
+def place_random_grid(species_counts, box, seed, dmin_site=1.2, dmin_heavy=2.5, max_tries=2000000):
+    """As place_random, with preallocated site arrays and the overlap test
+    vectorised over all placed sites (for boxes of thousands of sites)."""
+    rng = np.random.default_rng(seed)
+    box = np.asarray(box, dtype=float)
+    total = sum(n * sp.natoms for sp, n in species_counts)
+    pos = np.zeros((total, 3))
+    hv = np.zeros(total, dtype=bool)
+    m = 0
+    out = []
+    tries = 0
+    for sp, n in species_counts:
+        ref = sp.ref_coords - com_of(sp, sp.ref_coords)
+        heavy = np.array([a.eps_K > 0.0 for a in sp.atoms])
+        mols = np.zeros((n, sp.natoms, 3))
+        k = 0
+        while k < n:
+            tries += 1
+            if tries > max_tries:
+                raise RuntimeError("could not place molecules; lower the density")
+            c = (rng.random(3) - 0.5) * box
+            R = random_rotation(rng) if sp.natoms > 1 else np.eye(3)
+            xyz = ref @ R.T + c
+            if m:
+                d = xyz[:, None, :] - pos[None, :m, :]
+                d -= box * np.round(d / box)
+                r2 = np.einsum("ijk,ijk->ij", d, d)
+                if (r2 < dmin_site ** 2).any():
+                    continue
+                if (r2[heavy][:, hv[:m]] < dmin_heavy ** 2).any():
+                    continue
+            mols[k] = xyz
+            pos[m:m + sp.natoms] = xyz
+            hv[m:m + sp.natoms] = heavy
+            m += sp.natoms
+            k += 1
+        out.append(mols)
+    return out
+
+
 def place_random(species_counts, box, seed, dmin_site=1.2, dmin_heavy=2.5, max_tries=2000000):
     """Uniform random COM positions and uniform orientations with overlap
     rejection under the minimum image (seeded; not the reference's Make_Config)."""

@@ -52,7 +52,8 @@ enum { CASS_MOVE_TRANSLATION = 0, CASS_MOVE_ROTATION = 1, CASS_MOVE_INTRA = 2,
        CASS_MOVE_INSERTION = 3, CASS_MOVE_DELETION = 4 };
 /* arithmetic of the pair and k-space terms */
 enum { CASS_PREC_FP64 = 0,     /* plain FP64 everywhere (reference parity) */
-       CASS_PREC_MIXED = 1 };  /* FP32 pair/k terms, FP64 per-molecule and total sums */
+       CASS_PREC_MIXED = 1,    /* FP32 real-space q-q pair terms, FP64 LJ/Mie, FP64 sums */
+       CASS_PREC_FP32 = 2 };   /* FP32 for both pair terms, FP64 sums (fastest, ~1e-5 rel) */
 
 /* --------------------------------------------------------- box parameters
  * The box geometry and the VDW_Style / Charge_Style / Rcutoff_Low /
@@ -225,6 +226,9 @@ int cass_trial_plan_create(cass_engine *eng, int32_t nsets, const int32_t *box_o
 int cass_trial_plan_run(cass_trial_plan *plan, const double *trial_xyz, int32_t use_graph,
                         double *nrg, int32_t *overlap);
 int cass_trial_plan_autotune(cass_trial_plan *plan, const double *trial_xyz, int32_t *block);
+/* Per-set species (same site count) and/or excluded molecule for later runs. */
+int cass_trial_plan_update(cass_trial_plan *plan, const int32_t *species_of_set,
+                           const int32_t *exclude_mol);
 int cass_trial_plan_destroy(cass_trial_plan *plan);
 
 /* Compute_System_Total_Energy (energy_routines.f90:2648-2953) with intra

@@ -74,6 +74,10 @@ def widom_box(case, probe, prec):
     return fx, system, eng, box, e
 
 
+def probe_charged(system):
+    return any(a.charge != 0.0 for a in system.species[-1].atoms)
+
+
 @pytest.mark.parametrize("case,probe", WIDOM_CASES)
 def test_trial_energies_fp64(case, probe):
     """cass_trial_energies (the CBMC/Widom trial-set kernel) at the reference's
@@ -88,7 +92,9 @@ def test_trial_energies_fp64(case, probe):
         ok = ~ov
         ref = fx["energies"][ok, col]
         assert np.max(np.abs(nrg[ok] - ref) / np.maximum(np.abs(ref), 1.0)) < REL
-    if system.charge_style == "ewald":
+    # the reference forms the reciprocal term only for a charged probe
+    # (widom_insert.f90:236-240, 426-432); its column is 0 otherwise
+    if system.charge_style == "ewald" and probe_charged(system):
         er = box.ewald_widom(sp, place)
         ok = ~fx["overlap"][:, 0]
         ref = fx["energies"][ok, 4]
@@ -111,7 +117,7 @@ def test_widom_kernel_placements(case, probe):
     assert np.max(np.abs(eo[ok, 0] - ref[ok, 0]) / np.maximum(np.abs(ref[ok, 0]), 1.0)) < REL
     okc = ~fx["overlap"][:, 1] & ok
     assert np.max(np.abs(eo[okc, 1] - ref[okc, 5]) / np.maximum(np.abs(ref[okc, 5]), 1.0)) < REL
-    if system.charge_style == "ewald":
+    if system.charge_style == "ewald" and probe_charged(system):
         de_ref = ref[ok, 4] - e["reciprocal"]
         assert np.max(np.abs(eo[ok, 2] - de_ref) / np.maximum(np.abs(ref[ok, 4]), 1.0)) < REL
     # mixed precision against FP64
@@ -122,6 +128,14 @@ def test_widom_kernel_placements(case, probe):
     err = np.abs(d32 - d64) / np.maximum(np.abs(d64), 100.0)
     print(f"[gpu] mixed dE rel err {case}: max {err.max():.2e} median {np.median(err):.2e}", flush=True)
     assert err.max() < 1e-6
+    assert np.array_equal(eo2[:, 3], eo[:, 3])
+    # all-FP32 pair terms: the LJ s^12 - s^6 cancellation near r = sigma
+    # costs up to ~1e-5 of dE; bounded at 1e-4
+    eng.set_precision(E.FP32)
+    r3, wv3, eo3 = box.widom_placements(sp, place)
+    err3 = np.abs(eo3[ok, 0] + eo3[ok, 2] - d64) / np.maximum(np.abs(d64), 100.0)
+    print(f"[gpu] fp32 dE rel err {case}: max {err3.max():.2e} median {np.median(err3):.2e}", flush=True)
+    assert err3.max() < 1e-4
 
 
 def test_ewald_move_commit_revert():
