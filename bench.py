@@ -7,8 +7,9 @@ cut_tail, Ewald 9 A / 1e-5, 2354 k-vectors).  One step = one batch of 1e6
 insertions, each a full trial energy: the inter energy at the full cutoff and
 the CBMC cutoff, the reciprocal-space change and widom_var = exp(-beta dE).
 
-value = trial energies/s of the batched mixed-precision path (FP32 pair
-terms, FP64 accumulation and FP64 reciprocal sums), inputs resident in HBM.
+value = trial energies/s of the batched mixed-precision path (pair terms on
+FP32 hardware reciprocal seeds refined in FP64, FP64 geometry, accumulation
+and reciprocal sums), inputs resident in HBM.
 Reported beside it: the batched FP64 path, the plain FP64 path that scores
 one insertion per launch, the mixed-vs-FP64 error of dE, NVT moves/s of the
 512-water chain, the roofline of the Widom kernel and the CPU oracle.
@@ -36,18 +37,18 @@ def widom_frame():
     return S.with_probe(frame, S.species_co2()).parsed()
 
 
-def algorithmic_flops(system, box_xyz, sp_index, n_sample, seed, engine_mod, eng, box):
-    """FP32 and FP64 flops per insertion from the per-unit weights of the
-    design (pair term: min-image + r^2 = 14, LJ = 12, Ewald real = 34 + exp(4);
-    k term per probe site per k = 12 (phase recurrence) + 6 per k), counting
-    the pairs inside the cutoff for a sample of insertions."""
+def pair_counts(system, sp_index, n_sample, seed):
+    """Average number of LJ and q-q site pairs inside the cutoff per insertion
+    of the probe, over a sample of uniform random insertions."""
     from cassandra_amd import systems as S
     probe = system.species[sp_index]
     rng = np.random.default_rng(seed)
     L = system.box
     frame = np.concatenate([c.reshape(-1, 3) for c in system.coords if c.size])
-    q = np.concatenate([[a.charge for a in sp.atoms] * n for sp, n in zip(system.species, system.nmols)])
-    lj = np.concatenate([[a.eps_K > 0 for a in sp.atoms] * n for sp, n in zip(system.species, system.nmols)])
+    q = np.concatenate([np.array([a.charge for a in sp.atoms] * n, dtype=float)
+                        for sp, n in zip(system.species, system.nmols)])
+    lj = np.concatenate([np.array([a.eps_K > 0 for a in sp.atoms] * n, dtype=bool)
+                         for sp, n in zip(system.species, system.nmols)])
     ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
     rc2 = system.rcut_vdw ** 2
     n_lj = n_q = 0
@@ -62,12 +63,38 @@ def algorithmic_flops(system, box_xyz, sp_index, n_sample, seed, engine_mod, eng
                 n_lj += int((inr & lj).sum())
             if at.charge != 0:
                 n_q += int((inr & (q != 0)).sum())
-    n_lj /= n_sample
-    n_q /= n_sample
-    K = box.ewald_setup()
-    f32 = 14 * (n_lj + n_q) / 2 + 12 * n_lj + 38 * n_q   # min-image shared by the two terms of a pair
-    f64 = 12 * probe.natoms * K + 6 * K
-    return f32, f64, n_lj, n_q, K
+    return n_lj / n_sample, n_q / n_sample
+
+
+# Algorithmic flops per unit (an FMA counts 2; sqrt, division and the exp of
+# the A&S erfc count 1, 1 and 4):
+#   pair geometry (difference, minimum image, r^2)            11 per pair
+#   LJ 12-6 (s^2, s^6, s^12, 4 eps (s^12 - s^6))                8 per LJ pair
+#   Ewald real space (r, t = 1/(1 + p a r), A&S polynomial,
+#                     exp(-a^2 r^2), q_i q_j / r)              23 per q-q pair
+#   reciprocal change: phase step (complex multiply) and the
+#                     dC, dS sums                              10 per charged probe site per k
+#                     Cn (2 C dC + dC^2 + 2 S dS + dS^2)       10 per k
+FL_GEOM, FL_LJ, FL_EWALD, FL_KSITE, FL_K = 11.0, 8.0, 23.0, 10.0, 10.0
+
+
+def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K):
+    """(FP32, FP64) flops per insertion.  Pair geometry and the reciprocal
+    change are FP64 in every mode; the pair terms are FP32 in the fp32 mode
+    only (mixed refines FP32 hardware seeds in FP64)."""
+    from cassandra_amd import engine as E
+    pair = FL_LJ * n_lj + FL_EWALD * n_q
+    f64 = FL_GEOM * max(n_lj, n_q) + (FL_KSITE * n_probe_sites + FL_K) * K
+    f32 = 0.0
+    if prec == E.FP32:
+        f32 += pair
+    else:
+        f64 += pair
+    return f32, f64
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def time_widom(box, sp, n, seed, steps, warmup):
@@ -160,7 +187,18 @@ def main():
     ap.add_argument("--insertions", type=int, default=N_INS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the per-launch and NVT side measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        x = torch.ones(a.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4}))
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported here "
               "(insertion ranges would shard across GPUs with one all-reduce of the sums).")
@@ -171,66 +209,90 @@ def main():
     sp = len(system.species) - 1
     eng = E.Engine(system, precision=E.MIXED)
     box = eng.add_box(system, max_mols=sum(system.nmols) + 1)
-    box.ewald_setup()
+    K = box.ewald_setup()
     box.compute_system_total_energy()
     probe = system.species[sp]
     box.widom_set_template(sp, probe.ref_coords)
     n = a.insertions
+    log(f"frame {sum(system.nmols)} molecules, {K} k-vectors, {n} insertions per step")
 
     # batched mixed precision: the measured path
     eng.set_precision(E.MIXED)
     dt_mixed, k_mixed = time_widom(box, sp, n, 1234, a.steps, a.warmup)
     value = n * a.steps / dt_mixed
-    # batched FP64
+    log(f"mixed: {value:.4g} trial energies/s, kernel {k_mixed:.3f} ms")
+    # batched FP64 and all-FP32 pair terms
     eng.set_precision(E.FP64)
-    dt64, k64 = time_widom(box, sp, n, 1234, max(1, a.steps // 2), 1)
-    fp64_rate = n * max(1, a.steps // 2) / dt64
-    # accuracy of the mixed path against FP64 on the same insertions
+    s2 = max(1, a.steps // 2)
+    dt64, k64 = time_widom(box, sp, n, 1234, s2, 1)
+    fp64_rate = n * s2 / dt64
+    log(f"fp64: {fp64_rate:.4g} trial energies/s, kernel {k64:.3f} ms")
+    eng.set_precision(E.FP32)
+    dt32, k32 = time_widom(box, sp, n, 1234, s2, 1)
+    fp32_rate = n * s2 / dt32
+    log(f"fp32: {fp32_rate:.4g} trial energies/s, kernel {k32:.3f} ms")
+    # accuracy against FP64 on the same insertions
     m = 200000
     eng.set_precision(E.FP64)
     r64, w64 = box.widom_batch(sp, m, 777, keep=True)
-    eng.set_precision(E.MIXED)
-    r32, w32 = box.widom_batch(sp, m, 777, keep=True)
-    ok = (w64 > 0) & (w32 > 0)
-    beta = system.beta if hasattr(system, "beta") else 1.0
-    de_err = np.abs(np.log(w32[ok]) - np.log(w64[ok])) / beta           # |dE32 - dE64|
-    de64 = np.abs(np.log(w64[ok])) / beta
-    rel_de = de_err / np.maximum(de64, 100.0)
-    mu_rel = abs(r32.widom_sum - r64.widom_sum) / abs(r64.widom_sum)
+    errs = {}
+    for name, prec in (("mixed", E.MIXED), ("fp32", E.FP32)):
+        eng.set_precision(prec)
+        r32, w32 = box.widom_batch(sp, m, 777, keep=True)
+        ok = (w64 > 0) & (w32 > 0)
+        de_err = np.abs(np.log(w32[ok]) - np.log(w64[ok])) / system.beta     # |dE - dE_fp64|
+        de64 = np.abs(np.log(w64[ok])) / system.beta
+        rel_de = de_err / np.maximum(de64, 100.0)
+        errs[name] = {"max": float(rel_de.max()), "median": float(np.median(rel_de)),
+                      "widom_sum_rel": float(abs(r32.widom_sum - r64.widom_sum) / abs(r64.widom_sum)),
+                      "overlap_flags_equal": bool(np.array_equal(w32 > 0, w64 > 0))}
+    log(f"dE errors {errs}")
 
-    f32, f64, n_lj, n_q, K = algorithmic_flops(system, None, sp, 200, 5, E, eng, box)
-    t_ins = k_mixed * 1e-3 / n
-    achieved = (f32 + f64) / t_ins
-    t_ideal = f32 / FP32_PEAK + f64 / FP64_PEAK
-    peak = (f32 + f64) / t_ideal
-    roofline = {"bound": "mfma", "compute": "valu (no MFMA: gather/ALU sums)", "achieved": achieved / 1e12,
-                "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "kernel": "k_widom<float>", "kernel_ms": k_mixed,
-                "flops_per_insertion": {"fp32": f32, "fp64": f64}, "pairs_in_cutoff": {"lj": n_lj, "qq": n_q},
-                "kvectors": K}
+    n_lj, n_q = pair_counts(system, sp, 200, 5)
+    n_charged = sum(1 for at in probe.atoms if at.charge != 0)
+
+    def roof(prec, kernel_ms, label):
+        f32, f64 = algorithmic_flops(prec, n_lj, n_q, n_charged, K)
+        t_ins = kernel_ms * 1e-3 / n
+        achieved = (f32 + f64) / t_ins
+        peak = (f32 + f64) / (f32 / FP32_PEAK + f64 / FP64_PEAK)
+        return {"bound": "mfma", "compute": "VALU (MFMA unused: gather and scalar-pair sums)",
+                "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": None, "kernel": label, "kernel_ms": kernel_ms,
+                "flops_per_insertion": {"fp32": f32, "fp64": f64}}
+
+    roofline = roof(E.MIXED, k_mixed, "k_widom_sorted<PrecMixed>")
+    roofline["pairs_in_cutoff"] = {"lj": n_lj, "qq": n_q}
+    roofline["kvectors"] = K
 
     extra = {}
     if not a.quick:
         extra["plain_fp64_per_launch_trials_per_s"] = plain_fp64_per_launch(system, sp, 400, 3)
+        log(f"plain fp64 per launch: {extra['plain_fp64_per_launch_trials_per_s']:.4g} trials/s")
         extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(1000)
+        log(f"nvt: {extra['nvt_spce512_moves_per_s']:.4g} moves/s")
     cpu = None
     if not a.no_cpu:
         rate, ns, dt = cpu_baseline(system, sp, seconds=8.0)
         cpu = {"value": rate, "unit": "trial energies/s", "cores": 1, "kind": "port",
                "sample": f"{ns} CO2 insertions into the 512-water frame ({dt:.1f} s, oracle/liboracle.so, "
                          f"FP64, one thread on {platform.processor() or platform.machine()})"}
+        log(f"cpu oracle: {rate:.4g} trial energies/s")
     out = {
-        "metric": "trial energies/s & MC moves/s, batched mixed-prec vs plain FP64 GPU, ΔE<1e-6 rel",
+        "metric": "trial energies/s & MC moves/s, batched mixed-prec vs plain FP64 GPU, \u0394E<1e-6 rel",
         "value": value, "unit": "trial energies/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": dt_mixed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "fp32-pair/fp64-accumulate", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64 (pair terms on f32 hardware seeds)", "data": "synthetic",
         "config": {"workload": "Widom CO2 -> 512 SPC/E water frame, 1e6 insertions per step",
                    "insertions_per_step": n, "kvectors": K, "frame_sites": int(system.nsites())},
         "batched_fp64_trials_per_s": fp64_rate,
+        "batched_fp32_pair_trials_per_s": fp32_rate,
         "speedup_mixed_vs_batched_fp64": value / fp64_rate,
-        "mixed_dE_rel_err": {"max": float(rel_de.max()), "median": float(np.median(rel_de)),
-                             "widom_sum_rel": float(mu_rel)},
-        "roofline": roofline, "cpu_baseline": cpu, **extra,
+        "dE_rel_err_vs_fp64": errs,
+        "roofline": roofline,
+        "roofline_fp64": roof(E.FP64, k64, "k_widom_sorted<PrecFP64>"),
+        "roofline_fp32": roof(E.FP32, k32, "k_widom_sorted<PrecFP32>"),
+        "cpu_baseline": cpu, **extra,
     }
     if "plain_fp64_per_launch_trials_per_s" in extra:
         out["speedup_vs_plain_fp64_per_launch"] = value / extra["plain_fp64_per_launch_trials_per_s"]

@@ -72,6 +72,12 @@ struct cass_engine {
 };
 
 // host helpers shared between translation units
+#define HIP_TRY(e, call)                                          \
+  do {                                                            \
+    hipError_t st_ = (call);                                      \
+    if (st_ != hipSuccess) return cass_check_hip((e), st_, #call); \
+  } while (0)
+
 int cass_fail(cass_engine *e, int code, const std::string &msg);
 int cass_check_hip(cass_engine *e, hipError_t st, const char *what);
 void *cass_scratch(cass_engine *e, size_t bytes);

@@ -12,11 +12,6 @@
 
 using namespace cass;
 
-#define HIP_TRY(e, call)                                          \
-  do {                                                            \
-    hipError_t st_ = (call);                                      \
-    if (st_ != hipSuccess) return cass_check_hip((e), st_, #call); \
-  } while (0)
 
 int cass_fail(cass_engine *e, int code, const std::string &msg) {
   if (e) e->err = msg;
@@ -900,75 +895,13 @@ extern "C" int cass_ewald_widom(cass_engine *e, int32_t box, int32_t species, in
 // ----------------------------------------------------------------- Widom
 int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs &w);  // host.cpp
 
-static int run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed,
-                     const double *place, double *widom_var, double *eout, cass_widom_result *res) {
-  cass_box_state *b;
-  int rc = box_of(e, box, &b);
-  if (rc) return rc;
-  if (species < 0 || species >= e->nspecies || n_ins <= 0) return CASS_ERR_ARG;
-  if (e->sp_natoms[species] > kMaxWidomSites) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
-  WidomArgs w;
-  if ((rc = cass_widom_prepare(e, *b, species, w))) return rc;
-  w.n_ins = n_ins;
-  w.seed = seed;
-  const int nblk = (int)((n_ins + kWidomBlock - 1) / kWidomBlock);
-  const size_t ni = e->sp_natoms[species];
-  size_t bytes = sizeof(double) * nblk + sizeof(int64_t) * nblk + 64;
-  double *d_place = nullptr, *d_var = nullptr, *d_eout = nullptr;
-  char *s = (char *)cass_scratch(e, bytes);
-  if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
-  double *part = (double *)s;
-  int64_t *part_ov = (int64_t *)(s + sizeof(double) * nblk);
-  double *d_sum = (double *)(s + (sizeof(double) + sizeof(int64_t)) * nblk);
-  int64_t *d_nov = (int64_t *)(d_sum + 1);
-  if (place) {
-    HIP_TRY(e, hipMallocAsync((void **)&d_place, sizeof(double) * n_ins * ni * 3, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_place, place, sizeof(double) * n_ins * ni * 3, hipMemcpyHostToDevice, e->stream));
-  }
-  if (widom_var) HIP_TRY(e, hipMallocAsync((void **)&d_var, sizeof(double) * n_ins, e->stream));
-  if (eout) HIP_TRY(e, hipMallocAsync((void **)&d_eout, sizeof(double) * 4 * n_ins, e->stream));
-  w.place = d_place;
-  BoxDev bv = box_view(e, *b);
-  TopoDev tv = topo_view(e);
-  EwaldDev ew = ewald_view(*b, b->rows ? row_k0_of(*b) : nullptr);
-  if (!b->ewald_ready) {
-    ew.nvecs = 0;
-    ew.nrows = 0;
-  }
-  PairConst<double> cd = make_pair_const<double>(b->p, e->ntypes);
-  HIP_TRY(e, hipEventRecord(e->ev0, e->stream));
-  PairConst<float> cf = make_pair_const<float>(b->p, e->ntypes);
-  auto kern = e->prec == CASS_PREC_MIXED  ? k_widom<PrecMixed>
-              : e->prec == CASS_PREC_FP32 ? k_widom<PrecFP32>
-                                          : k_widom<PrecFP64>;
-  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kWidomBlock), 0, e->stream, w, bv, tv, ew, cd, cf, part, part_ov, d_var,
-                     d_eout);
-  HIP_TRY(e, hipGetLastError());
-  HIP_TRY(e, hipEventRecord(e->ev1, e->stream));
-  hipLaunchKernelGGL(k_sum_widom, dim3(1), dim3(256), 0, e->stream, part, part_ov, nblk, d_sum, d_nov);
-  HIP_TRY(e, hipGetLastError());
-  double hs[2];
-  HIP_TRY(e, hipMemcpyAsync(hs, d_sum, sizeof(double) * 2, hipMemcpyDeviceToHost, e->stream));
-  if (widom_var) HIP_TRY(e, hipMemcpyAsync(widom_var, d_var, sizeof(double) * n_ins, hipMemcpyDeviceToHost, e->stream));
-  if (eout) HIP_TRY(e, hipMemcpyAsync(eout, d_eout, sizeof(double) * 4 * n_ins, hipMemcpyDeviceToHost, e->stream));
-  if (d_place) HIP_TRY(e, hipFreeAsync(d_place, e->stream));
-  if (d_var) HIP_TRY(e, hipFreeAsync(d_var, e->stream));
-  if (d_eout) HIP_TRY(e, hipFreeAsync(d_eout, e->stream));
-  HIP_TRY(e, hipStreamSynchronize(e->stream));
-  float ms = 0.0f;
-  HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
-  e->last_kernel_ms = ms;
-  res->widom_sum = hs[0];
-  std::memcpy(&res->n_overlaps, &hs[1], sizeof(int64_t));
-  res->n_insertions = n_ins;
-  res->e_const = w.e_const;
-  return CASS_OK;
-}
+int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed, const double *place,
+                   double *widom_var, double *eout, cass_widom_result *res);  // widom.hip
 
 extern "C" int cass_widom_batch(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed,
                                 double *widom_var, cass_widom_result *res) {
   if (!res) return CASS_ERR_ARG;
-  return run_widom(e, box, species, n_ins, seed, nullptr, widom_var, nullptr, res);
+  return cass_run_widom(e, box, species, n_ins, seed, nullptr, widom_var, nullptr, res);
 }
 
 // Per-insertion body of Widom_Insert at explicit placements (test entry: the
@@ -978,5 +911,5 @@ extern "C" int cass_widom_placements(cass_engine *e, int32_t box, int32_t specie
                                      const double *xyz, double *widom_var, double *eout,
                                      cass_widom_result *res) {
   if (!res || !xyz) return CASS_ERR_ARG;
-  return run_widom(e, box, species, n, 0, xyz, widom_var, eout, res);
+  return cass_run_widom(e, box, species, n, 0, xyz, widom_var, eout, res);
 }

@@ -262,6 +262,8 @@ int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs
   bool charged = false;
   for (int a = 0; a < n; ++a) {
     w.q[a] = e->atom_charge[o + a];
+    w.mass[a] = e->atom_mass[o + a];
+    w.type[a] = e->atom_type[o + a];
     charged = charged || w.q[a] != 0.0;
   }
   w.charged = charged && b.p.charge_style == CASS_Q_EWALD && b.ewald_ready;

@@ -9,7 +9,6 @@
 namespace cass {
 
 constexpr int kMaxBlock = 256;        // block size of the reduction kernels
-constexpr int kWidomBlock = 256;      // insertions per Widom block
 constexpr int kMaxSites = 64;         // sites per molecule (trial kernels)
 constexpr int kMaxWidomSites = 8;     // sites per Widom test molecule
 
@@ -51,6 +50,10 @@ struct WidomArgs {
   double beta, e_const;
   double ref[kMaxWidomSites * 3];   // site offsets from the COM (reference geometry)
   double q[kMaxWidomSites];
+  double mass[kMaxWidomSites];
+  int type[kMaxWidomSites];         // atom type of each site (-1: none)
+  int cutoff_only;                  // every pair term needs r <= rc_pair
+  double rc_pair, rc_pair2;
   const double *place;              // explicit coordinates [n_ins][natoms][3] or null
 };
 
@@ -88,20 +91,29 @@ __device__ inline bool molpair_in_range(const BoxDev &b, bool cbmc, double4 ci, 
 }
 
 // Arithmetic of the two pair terms per precision mode: V for vdW, Q for the
-// real-space q-q term.  Mixed keeps LJ in FP64 (s^12 - s^6 cancels near
-// r = sigma, where FP32 loses ~1e-5 of the pair energy) and moves the
-// transcendental-heavy Coulomb term (rcp, sqrt, exp of the A&S erfc) to FP32.
+// real-space q-q term, M for the operation sequence.
+//   FP64:  the reference's operations in double (parity path).
+//   Mixed: double arithmetic on FP32 hardware seeds (rcp, rsq) refined by one
+//          Newton step, ~1e-14 relative per pair: plain FP32 pair terms lose
+//          ~1e-7 per pair, which the cancellation of LJ s^12 - s^6 near
+//          r = sigma and of the Coulomb sums amplifies to ~1e-5 of dE.
+//   FP32:  single-precision pair terms on the hardware seeds.
+// Pair geometry (minimum image, r^2, cutoff and overlap tests) is FP64 in
+// every mode, so all modes see the same pair set.
 struct PrecFP64 {
   using V = double;
   using Q = double;
+  using M = MathRef;
 };
 struct PrecMixed {
   using V = double;
-  using Q = float;
+  using Q = double;
+  using M = MathFast;
 };
 struct PrecFP32 {
   using V = float;
   using Q = float;
+  using M = MathFast;
 };
 
 template <typename T>
@@ -151,11 +163,11 @@ __device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const 
       TQ e_q = (TQ)0;
       if (ti >= 0 && tj >= 0 && (gv || (ecbmc && r2d <= cd.rc_cbmc2))) {
         const int k = ti * tp.ntypes + tj;
-        e_v = vdw_inter<TV>(cv, (TV)r2d, (TV)tp.eps[k], (TV)tp.sig[k], (TV)tp.mn[k], (TV)tp.mm[k]);
+        e_v = vdw_inter<TV, typename P::M>(cv, (TV)r2d, (TV)tp.eps[k], (TV)tp.sig[k], (TV)tp.mn[k], (TV)tp.mm[k]);
       }
       const double qj = tp.atom_charge[oj + ja];
       const bool any_q = cd.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
-      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<TQ>(cq, (TQ)r2d, (TQ)(qi * qj));
+      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<TQ, typename P::M>(cq, (TQ)r2d, (TQ)(qi * qj));
       if (gv) ev += (double)e_v;
       if (gq) eq += (double)e_q;
       if (ecbmc && r2d <= cd.rc_cbmc2) {
@@ -248,63 +260,6 @@ __host__ __device__ inline cplx cpow_int(cplx e, int n) {
   return r;
 }
 
-// Reciprocal-space energy change of adding a test molecule:
-// sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2) * charge_factor, with
-// exp(i h.r) = Ea^nx Eb^ny Ec^nz built by phase recurrence along the k rows
-// (h.r = 2 pi n.s with s the fractional coordinates, so this holds for the
-// k sets of both branches of Ewald_Reciprocal_Lattice_Vector_Setup).
-__device__ inline double widom_recip_delta(const EwaldDev &ew, int n, const double *px, const double *py,
-                                           const double *pz, const double *q, const BoxDev &b) {
-  cplx Ea[kMaxWidomSites], Eb[kMaxWidomSites], Ec[kMaxWidomSites], Pb[kMaxWidomSites], Pc[kMaxWidomSites];
-  for (int a = 0; a < n; ++a) {
-    const double sx = b.Li[0] * px[a] + b.Li[3] * py[a] + b.Li[6] * pz[a];
-    const double sy = b.Li[1] * px[a] + b.Li[4] * py[a] + b.Li[7] * pz[a];
-    const double sz = b.Li[2] * px[a] + b.Li[5] * py[a] + b.Li[8] * pz[a];
-    double s, c;
-    sincos(kTwoPi * sx, &s, &c);
-    Ea[a] = {c, s};
-    sincos(kTwoPi * sy, &s, &c);
-    Eb[a] = {c, s};
-    sincos(kTwoPi * sz, &s, &c);
-    Ec[a] = {c, s};
-  }
-  int cur_ny = -1000000, cur_nz = -1000000;
-  double de = 0.0;
-  for (int r = 0; r < ew.nrows; ++r) {
-    const int4 row = ew.rows[r];
-    const int ny = row.x, nz = row.y, nx0 = row.z, cnt = row.w;
-    if (nz != cur_nz) {
-      for (int a = 0; a < n; ++a) Pc[a] = cpow_int(Ec[a], nz);
-      cur_nz = nz;
-      cur_ny = -1000000;
-    }
-    if (ny == cur_ny + 1) {
-      for (int a = 0; a < n; ++a) Pb[a] = cmul(Pb[a], Eb[a]);
-    } else {
-      for (int a = 0; a < n; ++a) Pb[a] = cpow_int(Eb[a], ny);
-    }
-    cur_ny = ny;
-    cplx T[kMaxWidomSites];
-    for (int a = 0; a < n; ++a) {
-      cplx base = cmul(Pb[a], Pc[a]);
-      T[a] = nx0 == 0 ? base : cmul(base, cpow_int(Ea[a], nx0));
-    }
-    const int k0 = ew.row_k0[r];
-    for (int kk = 0; kk < cnt; ++kk) {
-      double dc = 0.0, ds = 0.0;
-      for (int a = 0; a < n; ++a) {
-        dc += q[a] * T[a].re;
-        ds += q[a] * T[a].im;
-        T[a] = cmul(T[a], Ea[a]);
-      }
-      const int k = k0 + kk;
-      const double c0 = ew.cs[k], s0 = ew.sn[k];
-      de += ew.cn[k] * (2.0 * c0 * dc + dc * dc + 2.0 * s0 * ds + ds * ds);
-    }
-  }
-  return de * kChargeFactor;
-}
-
 // kernels (kernels.hip)
 template <typename P>
 __global__ void k_trials(BoxDev b, TopoDev tp, PairConst<double> cd, PairConst<float> cf, int species, int ni,
@@ -321,9 +276,6 @@ __global__ void k_ewald_move(TopoDev tp, EwaldDev ew, int m, int species, int fl
                              double *part);
 __global__ void k_ewald_commit(EwaldDev ew, int m);
 __global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const double *txyz, double *part);
-template <typename P>
-__global__ void k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew, PairConst<double> cd,
-                        PairConst<float> cf, double *part, int64_t *part_ov, double *wvar, double *eout);
 __global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, double *out, int64_t *out_ov);
 
 }  // namespace cass

@@ -292,90 +292,6 @@ __global__ void k_ewald_widom(TopoDev tp, EwaldDev ew, int species, int n, const
 }
 
 // --------------------------------------------------------- Widom batch
-// The insertion loop of Widom_Insert (widom_insert.f90:319-489) for a rigid
-// single-fragment test molecule, one thread per insertion.  Placement from a
-// counter-based stream or from explicit coordinates; inter energy at the full
-// and the CBMC cutoff (overlap semantics of Build_Molecule,
-// fragment_growth.f90:519-544); reciprocal delta over the k rows by phase
-// recurrence in FP64; widom_var = exp(-beta (E_inter + dE_recip + e_const)).
-template <typename P>
-__global__ void __launch_bounds__(kWidomBlock) k_widom(WidomArgs w, BoxDev b, TopoDev tp, EwaldDev ew,
-                                                       PairConst<double> cd, PairConst<float> cf,
-                                                       double *part, int64_t *part_ov, double *wvar,
-                                                       double *eout) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ni = w.natoms;
-  double px[kMaxWidomSites], py[kMaxWidomSites], pz[kMaxWidomSites];
-  int pspa[kMaxWidomSites];
-  double v[1] = {0.0};
-  int64_t ovl = 0;
-  if (i < w.n_ins) {
-    if (w.place) {
-      for (int a = 0; a < ni; ++a) {
-        px[a] = w.place[(i * ni + a) * 3];
-        py[a] = w.place[(i * ni + a) * 3 + 1];
-        pz[a] = w.place[(i * ni + a) * 3 + 2];
-      }
-    } else {
-      place_probe(w, b, (uint64_t)i, px, py, pz);
-    }
-    for (int a = 0; a < ni; ++a) pspa[a] = w.spa0 + a;
-    const double4 pc = molecule_com(tp, w.spa0, ni, px, py, pz);
-    double ev = 0.0, eq = 0.0, ecb = 0.0;
-    bool over = false;
-    const bool coul_cbmc = tp.sp_coul_cbmc[w.species] != 0;
-    for (int j = 0; j < b.nmols && !over; ++j) {
-      if (!b.live[j]) continue;
-      const double4 cj = b.comd[j];
-      // the full-cutoff prefilter is the wider one; the CBMC sum is taken in
-      // the same pass for the pairs within rcut_cbmc.
-      if (!molpair_in_range(b, false, pc, cj) && !molpair_in_range(b, true, pc, cj)) continue;
-      over = molpair_energy<P>(b, tp, cd, cf, false, coul_cbmc, ni, px, py, pz, pspa, j, ev, eq, &ecb);
-    }
-    double e_inter = ev + eq;
-    double de_recip = 0.0;
-    if (!over && w.charged) de_recip = widom_recip_delta(ew, ni, px, py, pz, w.q, b);
-    const double dE = e_inter + de_recip + w.e_const;
-    double wv = 0.0;
-    // Build_Molecule marks a trial whose CBMC-cutoff energy reaches
-    // max_kBT as overlapped (fragment_growth.f90:536-539)
-    const bool cbmc_over = over || (w.beta * ecb >= kMaxKBT);
-    if (!cbmc_over) wv = exp(-w.beta * dE);
-    if (cbmc_over) ovl = 1;
-    v[0] = wv;
-    if (wvar) wvar[i] = wv;
-    if (eout) {
-      eout[4 * i] = e_inter;
-      eout[4 * i + 1] = ecb;
-      eout[4 * i + 2] = de_recip;
-      eout[4 * i + 3] = over ? 1.0 : 0.0;
-    }
-  }
-  // block reduction (sum of widom_var, count of overlaps)
-  __shared__ double shv[kWidomBlock / 64];
-  __shared__ int64_t sho[kWidomBlock / 64];
-  double s = wave_sum(v[0]);
-  int64_t o = ovl;
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) o += __shfl_down(o, d, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) {
-    shv[wid] = s;
-    sho[wid] = o;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0;
-    int64_t c = 0;
-    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
-      a += shv[k];
-      c += sho[k];
-    }
-    part[blockIdx.x] = a;
-    part_ov[blockIdx.x] = c;
-  }
-}
-
 __global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, double *out,
                             int64_t *out_ov) {
   double v[1] = {0.0};
@@ -403,9 +319,7 @@ __global__ void k_sum_widom(const double *part, const int64_t *part_ov, int n, d
   template __global__ void k_trials<P>(BoxDev, TopoDev, PairConst<double>, PairConst<float>, int, int,    \
                                        const double *, int, int, double *, int *);                        \
   template __global__ void k_total_pairs<P>(BoxDev, TopoDev, PairConst<double>, PairConst<float>,         \
-                                            double *, int *);                                             \
-  template __global__ void k_widom<P>(WidomArgs, BoxDev, TopoDev, EwaldDev, PairConst<double>,            \
-                                      PairConst<float>, double *, int64_t *, double *, double *);
+                                            double *, int *);
 CASS_INSTANTIATE(PrecFP64)
 CASS_INSTANTIATE(PrecMixed)
 CASS_INSTANTIATE(PrecFP32)

@@ -1,7 +1,8 @@
 // pair.h -- per-site-pair energy terms of the engine, templated on the
-// arithmetic type T of the pair math (double for the FP64 path, float for the
-// mixed path).  The caller forms the minimum-image separation in FP64 and
-// hands r^2 over in T; sums are accumulated by the caller in FP64.
+// arithmetic type T of the pair math and on the operation sequence M
+// (MathRef: the reference's operations; MathFast: hardware reciprocal
+// seeds).  The caller forms the minimum-image separation in FP64 and hands
+// r^2 over in T; sums are accumulated by the caller in FP64.
 //
 // Semantics follow Compute_AtomPair_Energy (energy_routines.f90:1831-2053),
 // Compute_AtomPair_Ewald_Real (:2088-2166) with the A&S 7.1.26 erfc
@@ -10,6 +11,8 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "../../include/cass_gpu.h"
 
@@ -65,14 +68,54 @@ __host__ __device__ inline float t_pow(float x, float y) { return powf(x, y); }
 __host__ __device__ inline double t_erfc(double x) { return erfc(x); }
 __host__ __device__ inline float t_erfc(float x) { return erfcf(x); }
 
-// A&S 7.1.26 erfc, energy_routines.f90:2143-2163.
+// Operation sequences of the pair math.  MathRef keeps the reference's own
+// operations (IEEE division and square root, libm exp).  MathFast takes the
+// gfx950 single-precision reciprocal / reciprocal-square-root instructions
+// (about 1 ulp of FP32) as seeds; in double one Newton step brings them to
+// ~1e-14 relative, in float they are used as they are.
+struct MathRef {};
+struct MathFast {};
+
+template <typename T, typename M>
+struct PM;
+
 template <typename T>
+struct PM<T, MathRef> {
+  __host__ __device__ static T rcp(T x) { return (T)1 / x; }
+  __host__ __device__ static T rsqrt(T x) { return (T)1 / t_sqrt(x); }
+  __host__ __device__ static T exp(T x) { return t_exp(x); }
+};
+
+template <>
+struct PM<double, MathFast> {
+  __device__ static double rcp(double x) {
+    const double y = (double)__builtin_amdgcn_rcpf((float)x);
+    const double e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+  }
+  __device__ static double rsqrt(double x) {
+    const double y = (double)__builtin_amdgcn_rsqf((float)x);
+    const double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+    return fma(0.5 * y, e, y);
+  }
+  __device__ static double exp(double x) { return ::exp(x); }
+};
+
+template <>
+struct PM<float, MathFast> {
+  __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  __device__ static float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+  __device__ static float exp(float x) { return __expf(x); }
+};
+
+// A&S 7.1.26 erfc, energy_routines.f90:2143-2163.
+template <typename T, typename M = MathRef>
 __host__ __device__ inline T erfc_as(T x) {
   const T A1 = (T)0.254829592, A2 = (T)-0.284496736, A3 = (T)1.421413741, A4 = (T)-1.453152027,
           A5 = (T)1.061405429, P = (T)0.3275911;
-  T t = (T)1 / ((T)1 + P * x);
+  T t = PM<T, M>::rcp((T)1 + P * x);
   T tp = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5))));
-  return tp * t_exp(-(x * x));
+  return tp * PM<T, M>::exp(-(x * x));
 }
 
 // Check_AtomPair_Cutoff (:3092-3195).
@@ -96,19 +139,32 @@ __host__ __device__ inline void pair_cutoff(const PairConst<T> &c, bool cbmc, T 
     gq = true;
 }
 
+// LJ 12-6 for the cut, cut_tail and cut_shift sums (inter branch of
+// :1908-1940).
+template <typename T, typename M = MathRef>
+__host__ __device__ inline T lj_cut(const PairConst<T> &c, T r2, T eps, T sig) {
+  T s2 = std::is_same<M, MathRef>::value ? (sig * sig) / r2 : (sig * sig) * PM<T, M>::rcp(r2);
+  T s6 = s2 * s2 * s2;
+  T s12 = s6 * s6;
+  T e = (T)4 * eps * (s12 - s6);
+  if (c.vdw_sum == CASS_SUM_CUT_SHIFT) {
+    T c2 = sig * sig / c.rc_vdw2, c6 = c2 * c2 * c2, c12 = c6 * c6;
+    e = e - (T)4 * eps * (c12 - c6);
+  }
+  return e;
+}
+
 // Intermolecular vdW pair energy (inter branch of :1908-2006).
-template <typename T>
+template <typename T, typename M = MathRef>
 __host__ __device__ inline T vdw_inter(const PairConst<T> &c, T r2, T eps, T sig, T mn, T mm) {
   if (c.vdw_style == CASS_VDW_LJ) {
-    T s2 = (sig * sig) / r2;
+    if (c.vdw_sum == CASS_SUM_CUT || c.vdw_sum == CASS_SUM_CUT_TAIL || c.vdw_sum == CASS_SUM_CUT_SHIFT)
+      return lj_cut<T, M>(c, r2, eps, sig);
+    T s2 = std::is_same<M, MathRef>::value ? (sig * sig) / r2 : (sig * sig) * PM<T, M>::rcp(r2);
     T s6 = s2 * s2 * s2;
     T s12 = s6 * s6;
     T e = (T)4 * eps * (s12 - s6);
     switch (c.vdw_sum) {
-      case CASS_SUM_CUT_SHIFT: {
-        T c2 = sig * sig / c.rc_vdw2, c6 = c2 * c2 * c2, c12 = c6 * c6;
-        e = e - (T)4 * eps * (c12 - c6);
-      } break;
       case CASS_SUM_CUT_SWITCH:
         if (r2 < c.ron2) {
         } else if (r2 <= c.roff2) {
@@ -147,29 +203,42 @@ __host__ __device__ inline T vdw_inter(const PairConst<T> &c, T r2, T eps, T sig
   return (T)0;
 }
 
-// Intermolecular q-q pair energy.  In FP64 the Ewald term keeps the
-// reference's form Eij - (1 - erfc)*Eij; in FP32 it is evaluated as the
-// mathematically equal Eij*erfc, which does not cancel.
-template <typename T>
-__host__ __device__ inline T qq_inter(const PairConst<T> &c, T r2, T qiqj) {
-  if (c.q_style == CASS_Q_EWALD) {
+// Real-space Ewald pair term (:2088-2166).  MathRef in FP64 keeps the
+// reference's form Eij - (1 - erfc)*Eij; otherwise the mathematically equal
+// Eij*erfc, which does not cancel.
+template <typename T, typename M = MathRef>
+__host__ __device__ inline T ewald_real(const PairConst<T> &c, T r2, T qiqj) {
+  constexpr bool ref64 = std::is_same<M, MathRef>::value && sizeof(T) == sizeof(double);
+  if (ref64) {
     T rij = t_sqrt(r2);
-    T ec = erfc_as(c.alpha * rij);
+    T ec = erfc_as<T, M>(c.alpha * rij);
     T eqq = qiqj / rij * (T)kChargeFactor;
-    if (sizeof(T) == sizeof(double)) {
-      T erf_val = (T)1 - ec;
-      return eqq - erf_val * eqq;
-    }
-    return eqq * ec;
+    T erf_val = (T)1 - ec;
+    return eqq - erf_val * eqq;
   }
+  T rinv = PM<T, M>::rsqrt(r2);
+  T ec = erfc_as<T, M>(c.alpha * (r2 * rinv));
+  return qiqj * rinv * (T)kChargeFactor * ec;
+}
+
+// Plain Coulomb (cut and minimum-image sums).
+template <typename T, typename M = MathRef>
+__host__ __device__ inline T coul_cut(T r2, T qiqj) {
+  constexpr bool ref64 = std::is_same<M, MathRef>::value && sizeof(T) == sizeof(double);
+  if (ref64) return (T)kChargeFactor * qiqj / t_sqrt(r2);
+  return (T)kChargeFactor * qiqj * PM<T, M>::rsqrt(r2);
+}
+
+// Intermolecular q-q pair energy.
+template <typename T, typename M = MathRef>
+__host__ __device__ inline T qq_inter(const PairConst<T> &c, T r2, T qiqj) {
+  if (c.q_style == CASS_Q_EWALD) return ewald_real<T, M>(c, r2, qiqj);
   if (c.q_style == CASS_Q_DSF) {
     T rij = t_sqrt(r2);
     T e = c.dsf2 * (rij - c.rc_coul) - c.dsf1 + t_erfc(c.alpha_dsf * rij) / rij;
     return e * (qiqj * (T)kChargeFactor);
   }
-  if (c.q_style == CASS_Q_CUT || c.q_style == CASS_Q_MINIMUM) {
-    return (T)kChargeFactor * qiqj / t_sqrt(r2);
-  }
+  if (c.q_style == CASS_Q_CUT || c.q_style == CASS_Q_MINIMUM) return coul_cut<T, M>(r2, qiqj);
   return (T)0;
 }
 

@@ -3,7 +3,7 @@
 ! cass_gpu_mod: 125 LJ sites on a jittered lattice in a cubic box (LJ
 ! cut_tail 8.5 A, 120 K / 3.4 A), total energy and one molecule energy.
 ! Prints "total <E>" and "mol1 <E_vdw>" for the test to compare with the
-! oracle.
+! oracle, and the first draws of the rranf stream ("rng1".."rng5").
 !------------------------------------------------------------------------------
 PROGRAM cass_fortran_check
   USE ISO_C_BINDING
@@ -13,6 +13,7 @@ PROGRAM cass_fortran_check
   TYPE(cass_topology_t) :: t
   TYPE(cass_energy_t) :: e
   TYPE(C_PTR) :: eng
+  TYPE(cass_rng_t) :: rng
   INTEGER(C_INT32_T) :: rc, box, ov
   INTEGER, PARAMETER :: n = 125
   REAL(C_DOUBLE), TARGET :: eps(1), sig(1), charge(1), mass(1), vsc(1), qsc(1)
@@ -97,5 +98,10 @@ PROGRAM cass_fortran_check
   WRITE(*, '(A,ES26.17E3)') 'total ', e%total
   WRITE(*, '(A,ES26.17E3)') 'mol1 ', evdw
   WRITE(*, '(A,ES26.17E3)') 'box ', L
+  ! the reference's rranf stream from the Fortran side (seeds 1234567, 7654321)
+  CALL cass_rng_init(rng, 1234567_C_INT64_T, 7654321_C_INT64_T)
+  DO i = 1, 5
+     WRITE(*, '(A,I0,A,ES26.17E3)') 'rng', i, ' ', cass_rng_uniform(rng)
+  END DO
   rc = cass_engine_destroy(eng)
 END PROGRAM cass_fortran_check

@@ -10,7 +10,7 @@ MODULE cass_gpu_mod
   IMPLICIT NONE
 
   INTEGER(C_INT32_T), PARAMETER :: CASS_OK = 0
-  INTEGER(C_INT32_T), PARAMETER :: CASS_PREC_FP64 = 0, CASS_PREC_MIXED = 1
+  INTEGER(C_INT32_T), PARAMETER :: CASS_PREC_FP64 = 0, CASS_PREC_MIXED = 1, CASS_PREC_FP32 = 2
   INTEGER(C_INT32_T), PARAMETER :: CASS_MOVE_TRANSLATION = 0, CASS_MOVE_ROTATION = 1, &
        CASS_MOVE_INTRA = 2, CASS_MOVE_INSERTION = 3, CASS_MOVE_DELETION = 4
 
@@ -34,6 +34,17 @@ MODULE cass_gpu_mod
      TYPE(C_PTR) :: species_natoms, species_atom_offset, species_pair_offset, species_coul_cbmc
      TYPE(C_PTR) :: atom_type, atom_charge, atom_mass, vdw_intra_scale, charge_intra_scale, bonded
   END TYPE cass_topology_t
+
+  TYPE, BIND(C) :: cass_widom_result_t
+     REAL(C_DOUBLE) :: widom_sum
+     INTEGER(C_INT64_T) :: n_overlaps, n_insertions
+     REAL(C_DOUBLE) :: e_const
+  END TYPE cass_widom_result_t
+
+  ! rranf state (random_generators.f90): five 64-bit LFSR components
+  TYPE, BIND(C) :: cass_rng_t
+     INTEGER(C_INT64_T) :: s(5)
+  END TYPE cass_rng_t
 
   TYPE, BIND(C) :: cass_energy_t
      REAL(C_DOUBLE) :: total, intra, inter, bond, angle, dihedral, improper
@@ -139,6 +150,37 @@ MODULE cass_gpu_mod
        REAL(C_DOUBLE), INTENT(IN) :: trial_xyz(*)
        REAL(C_DOUBLE), INTENT(OUT) :: nrg(*)
        INTEGER(C_INT32_T), INTENT(OUT) :: overlap(*)
+     END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_widom_set_template(eng, box, species, xyz, e_intra_const) &
+          BIND(C, NAME='cass_widom_set_template')
+       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box, species
+       REAL(C_DOUBLE), INTENT(IN) :: xyz(*)
+       REAL(C_DOUBLE), INTENT(OUT) :: e_intra_const
+     END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_widom_batch(eng, box, species, n_ins, seed, widom_var, res) &
+          BIND(C, NAME='cass_widom_batch')
+       IMPORT :: C_INT32_T, C_INT64_T, C_PTR, cass_widom_result_t
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box, species
+       INTEGER(C_INT64_T), VALUE :: n_ins, seed
+       TYPE(C_PTR), VALUE :: widom_var
+       TYPE(cass_widom_result_t), INTENT(OUT) :: res
+     END FUNCTION
+     SUBROUTINE cass_rng_init(r, i1, i3) BIND(C, NAME='cass_rng_init')
+       IMPORT :: C_INT64_T, cass_rng_t
+       TYPE(cass_rng_t), INTENT(OUT) :: r
+       INTEGER(C_INT64_T), VALUE :: i1, i3
+     END SUBROUTINE
+     REAL(C_DOUBLE) FUNCTION cass_rng_uniform(r) BIND(C, NAME='cass_rng_uniform')
+       IMPORT :: C_DOUBLE, cass_rng_t
+       TYPE(cass_rng_t), INTENT(INOUT) :: r
+     END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_accept_or_reject(r, ln_pacc) BIND(C, NAME='cass_accept_or_reject')
+       IMPORT :: C_INT32_T, C_DOUBLE, cass_rng_t
+       TYPE(cass_rng_t), INTENT(INOUT) :: r
+       REAL(C_DOUBLE), VALUE :: ln_pacc
      END FUNCTION
   END INTERFACE
 

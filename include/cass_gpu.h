@@ -277,17 +277,20 @@ int cass_ewald_widom(cass_engine *eng, int32_t box, int32_t species, int32_t ntr
  * single-fragment test species with kappa_ins = 1, all insertions in one
  * launch.  Placement u[0..5] per insertion comes from a counter-based stream
  * (seed, insertion index): COM = (u0,u1,u2) - 1/2 in box units and a uniform
- * rotation from (u3,u4,u5).  Per insertion: full-cutoff inter energy, the
- * intramolecular periodic Ewald term, the reciprocal energy with the test
- * molecule added, and widom_var = exp(-beta*dE) with
- * dE = E_inter + e_const (e_const = lrc_diff + E_self - E_recip_in + intra).
+ * rotation from (u3,u4,u5).  Per insertion: the inter energy at the full
+ * cutoff (and at rcut_cbmc for the max_kBT overlap test of Build_Molecule),
+ * the reciprocal energy change dE_recip of adding the molecule, and
+ * widom_var = exp(-beta*dE) with dE = E_inter + dE_recip + e_const
+ * (e_const = lrc_diff + E_self + intramolecular; the reference's
+ * E_reciprocal - E_recip_in is dE_recip).  Insertions are scored in a
+ * spatially sorted order; every value is written at its insertion index.
  * Sums are FP64.  If widom_var is non-NULL it receives every insertion's
  * value (n_ins doubles). */
 typedef struct cass_widom_result {
   double widom_sum;         /* sum of widom_var (prefactor V/(N Lambda^3) NOT applied) */
   int64_t n_overlaps;       /* insertions rejected by r^2 < rcut_low^2 */
   int64_t n_insertions;
-  double e_const;           /* lrc_diff + E_self - E_recip_in + intramolecular */
+  double e_const;           /* lrc_diff + E_self + intramolecular */
 } cass_widom_result;
 
 int cass_widom_batch(cass_engine *eng, int32_t box, int32_t species, int64_t n_ins,

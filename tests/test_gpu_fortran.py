@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from cassandra_amd import systems as S
-from tests.conftest import ROOT
+from tests.conftest import ROOT, golden_json
 
 pytestmark = pytest.mark.gpu
 EXE = os.path.join(ROOT, "cassandra_amd", "fortran", "cass_fortran_check")
@@ -43,3 +43,7 @@ def test_fortran_host_through_iso_c_binding(oracle_mod):
     v, q, _ = ob.mol_inter(0)
     assert abs(vals["total"] - e["total"]) <= 1e-10 * abs(e["total"])
     assert abs(vals["mol1"] - v) <= 1e-10 * max(1.0, abs(v))
+    # the reference's random stream through the same module (golden draws
+    # of seeds 1234567, 7654321 from the reference build)
+    draws = [float(x) for x in golden_json("rng.json")[0]["draws"][:5]]
+    assert [vals[f"rng{i}"] for i in range(1, 6)] == draws

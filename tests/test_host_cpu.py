@@ -106,3 +106,22 @@ def test_topology_tables():
                                                                + system.species[1].atoms[1].sigma) * 0.5) == 0
     vs = t.vdw_intra_scale[:9].reshape(3, 3)
     assert vs[0, 1] == 0.0 and vs[1, 2] == 0.0            # 1-2 and 1-3 of water excluded
+
+
+def test_cbmc_chain_growth_geometry():
+    """The CBMC driver's site placement (natural extension reference frame):
+    the new site sits at the bond length from c, at angle theta to b-c and
+    at dihedral phi about b-c, for broadcast batches of trials."""
+    from cassandra_amd import cbmc
+    r = np.random.default_rng(0)
+    a, b, c = [r.normal(size=(5, 3)) for _ in range(3)]
+    th = np.full(5, 1.9)
+    ph = r.uniform(-3.0, 3.0, size=(5, 7))
+    d = cbmc.place_next(a[:, None], b[:, None], c[:, None], 1.54, th[:, None], ph)
+    assert d.shape == (5, 7, 3)
+    assert np.allclose(np.linalg.norm(d - c[:, None], axis=-1), 1.54)
+    cb = b[:, None] - c[:, None]
+    cosang = (cb * (d - c[:, None])).sum(-1) / (np.linalg.norm(cb, axis=-1) * 1.54)
+    assert np.allclose(cosang, math.cos(1.9))
+    full = lambda x: np.broadcast_to(x[:, None], d.shape)
+    assert np.allclose(cbmc.cos_dihedral(full(a), full(b), full(c), d), np.cos(ph))
