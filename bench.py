@@ -79,18 +79,17 @@ FL_GEOM, FL_LJ, FL_EWALD, FL_KSITE, FL_K = 11.0, 8.0, 23.0, 10.0, 10.0
 
 
 def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K):
-    """(FP32, FP64) flops per insertion.  Pair geometry and the reciprocal
-    change are FP64 in every mode; the pair terms are FP32 in the fp32 mode
-    only (mixed refines FP32 hardware seeds in FP64)."""
+    """Per insertion: {"real": (FP32, FP64), "recip": (FP32, FP64)} flops of
+    the real-space pair kernel and of the reciprocal/combine kernel.  Pair
+    geometry and the reciprocal change are FP64 in every mode; the pair terms
+    are FP32 in the fp32 mode only (mixed refines FP32 hardware seeds in
+    FP64)."""
     from cassandra_amd import engine as E
     pair = FL_LJ * n_lj + FL_EWALD * n_q
-    f64 = FL_GEOM * max(n_lj, n_q) + (FL_KSITE * n_probe_sites + FL_K) * K
-    f32 = 0.0
-    if prec == E.FP32:
-        f32 += pair
-    else:
-        f64 += pair
-    return f32, f64
+    geom = FL_GEOM * max(n_lj, n_q)
+    real = (pair, geom) if prec == E.FP32 else (0.0, pair + geom)
+    recip = (0.0, (FL_KSITE * n_probe_sites + FL_K) * K)
+    return {"real": real, "recip": recip}
 
 
 def log(msg):
@@ -98,15 +97,18 @@ def log(msg):
 
 
 def time_widom(box, sp, n, seed, steps, warmup):
+    """Wall time of `steps` batches and the mean device time of each stage
+    (placement + sort, real space, reciprocal + combine, whole batch) from the
+    engine's HIP events."""
     for w in range(warmup):
         box.widom_batch(sp, n, seed + 1000 + w)
-    ks = []
+    st = []
     t0 = time.perf_counter()
     for s in range(steps):
         box.widom_batch(sp, n, seed + s)
-        ks.append(box.eng.last_kernel_ms())
+        st.append(box.eng.last_stage_ms())
     dt = time.perf_counter() - t0
-    return dt, float(np.mean(ks))
+    return dt, [float(x) for x in np.mean(np.array(st), axis=0)]
 
 
 def plain_fp64_per_launch(system, sp, n, seed):
@@ -187,6 +189,8 @@ def main():
     ap.add_argument("--insertions", type=int, default=N_INS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the per-launch and NVT side measurements")
+    ap.add_argument("--profile-mixed", action="store_true",
+                    help="only the timed mixed-precision steps (for a profiler run of the measured kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scale for the FETCH_SIZE/WRITE_SIZE counters)")
     a = ap.parse_args()
@@ -220,17 +224,22 @@ def main():
     eng.set_precision(E.MIXED)
     dt_mixed, k_mixed = time_widom(box, sp, n, 1234, a.steps, a.warmup)
     value = n * a.steps / dt_mixed
-    log(f"mixed: {value:.4g} trial energies/s, kernel {k_mixed:.3f} ms")
+    log(f"mixed: {value:.4g} trial energies/s, stages (place+sort, real, recip+combine, batch) "
+        f"{', '.join(f'{x:.3f}' for x in k_mixed)} ms")
+    if a.profile_mixed:
+        print(json.dumps({"value": value, "unit": "trial energies/s", "stage_ms": k_mixed, "steps": a.steps,
+                          "warmup": a.warmup}))
+        return
     # batched FP64 and all-FP32 pair terms
     eng.set_precision(E.FP64)
     s2 = max(1, a.steps // 2)
     dt64, k64 = time_widom(box, sp, n, 1234, s2, 1)
     fp64_rate = n * s2 / dt64
-    log(f"fp64: {fp64_rate:.4g} trial energies/s, kernel {k64:.3f} ms")
+    log(f"fp64: {fp64_rate:.4g} trial energies/s, stages {', '.join(f'{x:.3f}' for x in k64)} ms")
     eng.set_precision(E.FP32)
     dt32, k32 = time_widom(box, sp, n, 1234, s2, 1)
     fp32_rate = n * s2 / dt32
-    log(f"fp32: {fp32_rate:.4g} trial energies/s, kernel {k32:.3f} ms")
+    log(f"fp32: {fp32_rate:.4g} trial energies/s, stages {', '.join(f'{x:.3f}' for x in k32)} ms")
     # accuracy against FP64 on the same insertions
     m = 200000
     eng.set_precision(E.FP64)
@@ -251,17 +260,33 @@ def main():
     n_lj, n_q = pair_counts(system, sp, 200, 5)
     n_charged = sum(1 for at in probe.atoms if at.charge != 0)
 
-    def roof(prec, kernel_ms, label):
-        f32, f64 = algorithmic_flops(prec, n_lj, n_q, n_charged, K)
-        t_ins = kernel_ms * 1e-3 / n
+    def roof1(f32, f64, ms, label):
+        t_ins = ms * 1e-3 / n
         achieved = (f32 + f64) / t_ins
         peak = (f32 + f64) / (f32 / FP32_PEAK + f64 / FP64_PEAK)
         return {"bound": "mfma", "compute": "VALU (MFMA unused: gather and scalar-pair sums)",
                 "achieved": achieved / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s", "frac": achieved / peak,
-                "traffic": None, "kernel": label, "kernel_ms": kernel_ms,
+                "traffic": None, "kernel": label, "kernel_ms": ms,
                 "flops_per_insertion": {"fp32": f32, "fp64": f64}}
 
-    roofline = roof(E.MIXED, k_mixed, "k_widom_sorted<PrecMixed>")
+    def roof(prec, stages, tag):
+        """Roofline of the dominant kernel of the batch (by device time), the
+        other kernel and the whole batch."""
+        fl = algorithmic_flops(prec, n_lj, n_q, n_charged, K)
+        r_real = roof1(*fl["real"], stages[1], f"k_widom_real<{tag}>")
+        r_recip = roof1(*fl["recip"], stages[2], "k_widom_combine<3>")
+        tot32 = fl["real"][0] + fl["recip"][0]
+        tot64 = fl["real"][1] + fl["recip"][1]
+        r_all = roof1(tot32, tot64, stages[3], "whole batch (place, sort, real, combine)")
+        dom, other = (r_real, r_recip) if stages[1] >= stages[2] else (r_recip, r_real)
+        dom = dict(dom)
+        dom["other_kernel"] = other
+        dom["batch"] = r_all
+        dom["stage_ms"] = {"place_sort": stages[0], "real": stages[1], "recip_combine": stages[2],
+                           "batch": stages[3]}
+        return dom
+
+    roofline = roof(E.MIXED, k_mixed, "PrecMixed")
     roofline["pairs_in_cutoff"] = {"lj": n_lj, "qq": n_q}
     roofline["kvectors"] = K
 
@@ -290,8 +315,8 @@ def main():
         "speedup_mixed_vs_batched_fp64": value / fp64_rate,
         "dE_rel_err_vs_fp64": errs,
         "roofline": roofline,
-        "roofline_fp64": roof(E.FP64, k64, "k_widom_sorted<PrecFP64>"),
-        "roofline_fp32": roof(E.FP32, k32, "k_widom_sorted<PrecFP32>"),
+        "roofline_fp64": roof(E.FP64, k64, "PrecFP64"),
+        "roofline_fp32": roof(E.FP32, k32, "PrecFP32"),
         "cpu_baseline": cpu, **extra,
     }
     if "plain_fp64_per_launch_trials_per_s" in extra:

@@ -68,6 +68,9 @@ struct cass_engine {
   void *io = nullptr;               // device input/output staging of the small calls
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // brackets the last batched kernel
   double last_kernel_ms = 0.0;
+  // stage events of the last Widom batch: start, placed+sorted, real space, end
+  hipEvent_t evs[4] = {nullptr, nullptr, nullptr, nullptr};
+  double last_stage_ms[4] = {0.0, 0.0, 0.0, 0.0};
   size_t io_bytes = 0;
 };
 

@@ -164,7 +164,9 @@ extern "C" int cass_engine_create(int32_t device, const cass_topology *t, int32_
   e->device = device;
   e->max_boxes = max_boxes;
   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess) {
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreate(&e->evs[0]) != hipSuccess || hipEventCreate(&e->evs[1]) != hipSuccess ||
+      hipEventCreate(&e->evs[2]) != hipSuccess || hipEventCreate(&e->evs[3]) != hipSuccess) {
     delete e;
     return CASS_ERR_HIP;
   }
@@ -240,6 +242,8 @@ extern "C" int cass_engine_destroy(cass_engine *e) {
   if (e->pinned) (void)hipHostFree(e->pinned);
   (void)hipEventDestroy(e->ev0);
   (void)hipEventDestroy(e->ev1);
+  for (hipEvent_t ev : e->evs)
+    if (ev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(e->stream);
   delete e;
   return CASS_OK;
@@ -279,6 +283,12 @@ extern "C" const char *cass_last_error(cass_engine *e) { return e ? e->err.c_str
 extern "C" int cass_last_kernel_ms(cass_engine *e, double *ms) {
   if (!e || !ms) return CASS_ERR_ARG;
   *ms = e->last_kernel_ms;
+  return CASS_OK;
+}
+
+extern "C" int cass_last_stage_ms(cass_engine *e, double *ms) {
+  if (!e || !ms) return CASS_ERR_ARG;
+  for (int k = 0; k < 4; ++k) ms[k] = e->last_stage_ms[k];
   return CASS_OK;
 }
 

@@ -124,14 +124,25 @@ __device__ inline void place_probe_n(const WidomArgs &w, const BoxDev &b, uint64
   }
 }
 
+// Per-k weights of the reciprocal change: (Cn, 2 Cn C_k, 2 Cn S_k, 0) from
+// the box structure factor (C_k, S_k) = sum_sites q (cos, sin)(h.r).
+__global__ void k_widom_weights(EwaldDev ew, double4 *__restrict__ wk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ew.nvecs) return;
+  const double cn = ew.cn[k];
+  wk[k] = make_double4(cn, 2.0 * cn * ew.cs[k], 2.0 * cn * ew.sn[k], 0.0);
+}
+
 // Reciprocal-space energy change of adding the test molecule:
-// charge_factor * sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2) with
-// exp(i h.r) = Ea^nx Eb^ny Ec^nz built by phase recurrence along the k rows
-// (h.r = 2 pi n.s, s the fractional coordinates, for the k sets of both
-// branches of Ewald_Reciprocal_Lattice_Vector_Setup).  FP64 throughout.
+// charge_factor * sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2), written as
+// sum_k [Cn (dC^2 + dS^2) + (2 Cn C) dC + (2 Cn S) dS], with
+// q_a exp(i h.r_a) = q_a Ea^nx Eb^ny Ec^nz built by phase recurrence along
+// the k rows (h.r = 2 pi n.s, s the fractional coordinates, for the k sets of
+// both branches of Ewald_Reciprocal_Lattice_Vector_Setup).  FP64 throughout.
 template <int NS>
-__device__ inline double recip_delta_n(const EwaldDev &ew, const BoxDev &b, const WidomArgs &w,
-                                       const double (&px)[NS], const double (&py)[NS], const double (&pz)[NS]) {
+__device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__restrict__ wk, const BoxDev &b,
+                                       const WidomArgs &w, const double (&px)[NS], const double (&py)[NS],
+                                       const double (&pz)[NS]) {
   cplx Ea[NS], Eb[NS], Ec[NS], Pb[NS], Pc[NS];
 #pragma unroll
   for (int a = 0; a < NS; ++a) {
@@ -170,21 +181,25 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const BoxDev &b, cons
     cplx T[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
-      const cplx base = cmul(Pb[a], Pc[a]);
-      T[a] = nx0 == 0 ? base : cmul(base, cpow_int(Ea[a], nx0));
+      cplx base = cmul(Pb[a], Pc[a]);
+      if (nx0 != 0) base = cmul(base, cpow_int(Ea[a], nx0));
+      T[a] = {w.q[a] * base.re, w.q[a] * base.im};
     }
-    const int k0 = ew.row_k0[r];
+    const double4 *wr = wk + ew.row_k0[r];
+#pragma unroll 2
     for (int kk = 0; kk < cnt; ++kk) {
-      double dc = 0.0, ds = 0.0;
+      double dc = T[0].re, ds = T[0].im;
 #pragma unroll
-      for (int a = 0; a < NS; ++a) {
-        dc += w.q[a] * T[a].re;
-        ds += w.q[a] * T[a].im;
-        T[a] = cmul(T[a], Ea[a]);
+      for (int a = 1; a < NS; ++a) {
+        dc += T[a].re;
+        ds += T[a].im;
       }
-      const int k = k0 + kk;
-      const double c0 = ew.cs[k], s0 = ew.sn[k];
-      de += ew.cn[k] * (2.0 * c0 * dc + dc * dc + 2.0 * s0 * ds + ds * ds);
+#pragma unroll
+      for (int a = 0; a < NS; ++a) T[a] = cmul(T[a], Ea[a]);
+      const double4 c = wr[kk];
+      de = fma(c.x, fma(dc, dc, ds * ds), de);
+      de = fma(c.y, dc, de);
+      de = fma(c.z, ds, de);
     }
   }
   return de * kChargeFactor;
@@ -275,16 +290,371 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
   }
 }
 
+// ------------------------------------------------------------ split path
+// For the common styles the insertion energy is formed in three kernels:
+//  k_widom_place: sites of every insertion (counter stream or explicit) and a
+//    sort key per site = (site index in the molecule, cell of the site);
+//  k_widom_real: one lane per site, in sorted order, so the 64 lanes of a
+//    wavefront hold sites of one kind within one small cell: the pair code
+//    runs where some lane has the pair within the cutoff, and that region is
+//    barely wider than each lane's own;
+//  k_widom_combine: one lane per insertion: sums its sites' partials in site
+//    order, adds the reciprocal change and forms widom_var.
+// The site partials are (E_vdw, E_qq, E_cbmc, overlap) at [insertion][site].
+
+template <int NS>
+__global__ void __launch_bounds__(256) k_widom_place(WidomArgs w, BoxDev b, int g0, int g1, int g2,
+                                                     double *__restrict__ pos, unsigned *__restrict__ keys,
+                                                     int *__restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.n_ins) return;
+  double px[NS], py[NS], pz[NS];
+  if (w.place) {
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+      px[a] = w.place[(i * NS + a) * 3];
+      py[a] = w.place[(i * NS + a) * 3 + 1];
+      pz[a] = w.place[(i * NS + a) * 3 + 2];
+    }
+  } else {
+    place_probe_n<NS>(w, b, (uint64_t)i, px, py, pz);
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+      pos[(i * NS + a) * 3] = px[a];
+      pos[(i * NS + a) * 3 + 1] = py[a];
+      pos[(i * NS + a) * 3 + 2] = pz[a];
+    }
+  }
+  const unsigned ncell = (unsigned)(g0 * g1 * g2);
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    double fx = b.Li[0] * px[a] + b.Li[3] * py[a] + b.Li[6] * pz[a];
+    double fy = b.Li[1] * px[a] + b.Li[4] * py[a] + b.Li[7] * pz[a];
+    double fz = b.Li[2] * px[a] + b.Li[5] * py[a] + b.Li[8] * pz[a];
+    fx -= floor(fx + 0.5);
+    fy -= floor(fy + 0.5);
+    fz -= floor(fz + 0.5);
+    const int cx = min(g0 - 1, max(0, (int)((fx + 0.5) * g0)));
+    const int cy = min(g1 - 1, max(0, (int)((fy + 0.5) * g1)));
+    const int cz = min(g2 - 1, max(0, (int)((fz + 0.5) * g2)));
+    keys[i * NS + a] = (unsigned)a * ncell + (unsigned)((cz * g1 + cy) * g0 + cx);
+    vals[i * NS + a] = (int)(i * NS + a);
+  }
+}
+
+// Arguments of the real-space site kernel (cubic boxes, common styles).
+struct RealArgs {
+  int64_t nslot;
+  int ns, ntypes, nmols, q_style;  // q_style: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT
+  int coul_cbmc, nexp, same_cut;   // same_cut: rc_vdw = rc_coul = rc_pair
+  double rc_vdw2, rc_coul2, rc_cbmc2, rc_low2, rc_pair2, rc_pair, alpha;
+  double L[3], hL[3], Li[3];
+  double dcom_max;  // largest max_dcom of the frame molecules
+};
+
+__device__ inline double wave_uniform(double v) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
+
+// exp(z), z <= 0: exp(-k/32) from the table times a degree-5 Taylor factor
+// of the remainder |d| <= 1/64 (relative error < 3e-14).  Lanes past the
+// table take the library exp.
+__device__ inline double exp_tab(double z, const double *t, int n) {
+  const double kf = rint(-32.0 * z);
+  const double d = fma(kf, 0.03125, z);
+  double p = fma(d, 1.0 / 120.0, 1.0 / 24.0);
+  p = fma(d, p, 1.0 / 6.0);
+  p = fma(d, p, 0.5);
+  p = fma(d, p, 1.0);
+  p = fma(d, p, 1.0);
+  const int k = (int)kf;
+  if (wave_any(k >= n)) return k < n ? t[k] * p : exp(z);
+  return t[k] * p;
+}
+
+// One pair of the real-space site kernel: LJ (cut, cut_tail, cut_shift: the
+// table holds 4 eps, sigma^2 and the shift 4 eps (c^12 - c^6)) and the
+// Ewald or plain Coulomb term, in the precision mode's arithmetic.
+template <typename P>
+__device__ inline void site_pair_energy(const RealArgs &ra, double r2, double4 prm, double qq, const double *etab,
+                                        bool do_v, bool do_q, double &e_v, double &e_q) {
+  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027, A5 = 1.061405429,
+               PA = 0.3275911;
+  if constexpr (std::is_same<P, PrecFP64>::value) {
+    // the reference's operations (lj_cut / ewald_real with MathRef)
+    if (do_v) {
+      const double s2 = prm.y / r2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+      e_v = prm.x * (s12 - s6) - prm.z;
+    }
+    if (do_q) {
+      const double rij = sqrt(r2);
+      if (ra.q_style == CASS_Q_EWALD) {
+        const double ec = erfc_as<double, MathRef>(ra.alpha * rij);
+        const double eqq = qq / rij * kChargeFactor;
+        e_q = eqq - (1.0 - ec) * eqq;
+      } else {
+        e_q = kChargeFactor * qq / rij;
+      }
+    }
+  } else if constexpr (std::is_same<P, PrecMixed>::value) {
+    // FP64 on hardware seeds: v_rcp_f64 / v_rsq_f64 refined by one Newton
+    // step, exp from the table
+    if (do_v) {
+      double y = __builtin_amdgcn_rcp(r2);
+      y = fma(y, fma(-r2, y, 1.0), y);
+      const double s2 = prm.y * y, s6 = s2 * s2 * s2, s12 = s6 * s6;
+      e_v = prm.x * (s12 - s6) - prm.z;
+    }
+    if (do_q) {
+      double ri = __builtin_amdgcn_rsq(r2);
+      ri = fma(0.5 * ri, fma(-r2 * ri, ri, 1.0), ri);
+      if (ra.q_style == CASS_Q_EWALD) {
+        const double x = ra.alpha * (r2 * ri);
+        const double den = fma(PA, x, 1.0);
+        double t = __builtin_amdgcn_rcp(den);
+        t = fma(t, fma(-den, t, 1.0), t);
+        const double tp = t * fma(t, fma(t, fma(t, fma(t, A5, A4), A3), A2), A1);
+        e_q = qq * kChargeFactor * ri * (tp * exp_tab(-(x * x), etab, ra.nexp));
+      } else {
+        e_q = kChargeFactor * qq * ri;
+      }
+    }
+  } else {
+    // single precision on the hardware instructions
+    const float r2f = (float)r2;
+    if (do_v) {
+      const float s2 = (float)prm.y * __builtin_amdgcn_rcpf(r2f), s6 = s2 * s2 * s2, s12 = s6 * s6;
+      e_v = (double)((float)prm.x * (s12 - s6) - (float)prm.z);
+    }
+    if (do_q) {
+      const float ri = __builtin_amdgcn_rsqf(r2f);
+      if (ra.q_style == CASS_Q_EWALD) {
+        const float x = (float)ra.alpha * (r2f * ri);
+        const float t = __builtin_amdgcn_rcpf(fmaf((float)PA, x, 1.0f));
+        const float tp =
+            t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, (float)A5, (float)A4), (float)A3), (float)A2), (float)A1);
+        e_q = (double)((float)(qq * kChargeFactor) * ri * (tp * __expf(-(x * x))));
+      } else {
+        e_q = (double)((float)(kChargeFactor * qq) * ri);
+      }
+    }
+  }
+}
+
+// Pair terms of one site (lane) against frame molecule m.  UNI: every site of
+// m takes the image shift of m's centre seen from the wave centre c,
+// S = -L rint((c - com)/L).  When (wave extent + cutoff + max_dcom) is below
+// half the box, S is the per-pair minimum image of
+// minimum_image_separation.f90:35-60 for every pair inside the cutoff, and
+// any other pair stays outside the cutoff; (p - s) + S is then the
+// reference's own separation, bit for bit.
+template <typename P, bool UNI>
+__device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
+                                  const int *__restrict__ ftype, const double4 *tab, const double *etab, double cx,
+                                  double cy, double cz, bool live, int a, double qa, double px, double py, double pz,
+                                  double &ev, double &eq, double &ecb, bool &over) {
+  // the shift of the molecule's centre serves all its sites (see the UNI
+  // condition in k_widom_real)
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  if (UNI) {
+    sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
+    sy = -ra.L[1] * rint((cy - m.com.y) * ra.Li[1]);
+    sz = -ra.L[2] * rint((cz - m.com.z) * ra.Li[2]);
+  }
+  for (int k = m.first; k < m.first + m.n; ++k) {
+    const double4 st = fsite[k];
+    const int tj = ftype[k];
+    double dx = px - st.x, dy = py - st.y, dz = pz - st.z;
+    if (UNI) {
+      dx += sx;
+      dy += sy;
+      dz += sz;
+    } else {
+      if (dx > ra.hL[0]) dx -= ra.L[0]; else if (dx < -ra.hL[0]) dx += ra.L[0];
+      if (dy > ra.hL[1]) dy -= ra.L[1]; else if (dy < -ra.hL[1]) dy += ra.L[1];
+      if (dz > ra.hL[2]) dz -= ra.L[2]; else if (dz < -ra.hL[2]) dz += ra.L[2];
+    }
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    const bool near = live && r2 <= ra.rc_pair2;
+    if (!wave_any(near)) continue;
+    if (near) {
+      over = over || r2 < ra.rc_low2;
+      const bool incb = r2 <= ra.rc_cbmc2;
+      const double4 prm = tj >= 0 ? tab[a * ra.ntypes + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
+      const double qq = qa * st.w;
+      const bool gv = r2 <= ra.rc_vdw2, gq = r2 <= ra.rc_coul2;
+      const bool do_v = prm.x != 0.0 && (gv || incb);  // eps = 0 (or no type) adds exactly 0
+      const bool do_q = ra.q_style != CASS_Q_NONE && qq != 0.0 && (gq || incb);
+      double e_v = 0.0, e_q = 0.0;
+      site_pair_energy<P>(ra, r2, prm, qq, etab, do_v, do_q, e_v, e_q);
+      if (ra.same_cut) {
+        ev += e_v;
+        eq += e_q;
+      } else {
+        if (gv) ev += e_v;
+        if (gq) eq += e_q;
+      }
+      if (incb) ecb += e_v + (ra.coul_cbmc ? e_q : 0.0);
+    }
+  }
+}
+
+template <typename P>
+__global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const FrameMol *__restrict__ fmol,
+                                                        const double4 *__restrict__ fsite,
+                                                        const int *__restrict__ ftype, const double *__restrict__ pos,
+                                                        const int *__restrict__ order, const double4 *__restrict__ ptab,
+                                                        const double *__restrict__ site_q,
+                                                        const double *__restrict__ etab_g, double4 *__restrict__ out) {
+  extern __shared__ double4 sh[];  // [site][type] (4 eps, sigma^2, shift, 0), then the exp table
+  const int ntab = ra.ns * ra.ntypes;
+  double *etab = (double *)(sh + ntab);
+  for (int t = threadIdx.x; t < ntab; t += blockDim.x) sh[t] = ptab[t];
+  for (int t = threadIdx.x; t < ra.nexp; t += blockDim.x) etab[t] = etab_g[t];
+  __syncthreads();
+  const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  const bool active = s < ra.nslot;
+  const int v = order[active ? s : ra.nslot - 1];
+  const int a = v % ra.ns;
+  const double qa = site_q[a];
+  const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
+  double ev = 0.0, eq = 0.0, ecb = 0.0;
+  bool over = false;
+  bool dead = !active;
+  // wave centre (first lane's site) and the wave's extent around it
+  const double cx = wave_uniform(px), cy = wave_uniform(py), cz = wave_uniform(pz);
+  double dx = px - cx, dy = py - cy, dz = pz - cz;
+  const double ext = wave_max(fmax(fabs(dx), fmax(fabs(dy), fabs(dz))));
+  dx -= ra.L[0] * rint(dx * ra.Li[0]);
+  dy -= ra.L[1] * rint(dy * ra.Li[1]);
+  dz -= ra.L[2] * rint(dz * ra.Li[2]);
+  const double rad = sqrt(wave_max(dx * dx + dy * dy + dz * dz));
+  const bool uni = ext + ra.rc_pair + ra.dcom_max + 1e-3 < fmin(ra.hL[0], fmin(ra.hL[1], ra.hL[2]));
+  for (int base = 0; base < ra.nmols && wave_any(!dead); base += 64) {
+    const int jt = base + (int)(threadIdx.x & 63);
+    bool cand = false;
+    if (jt < ra.nmols) {
+      const double4 cj = fmol[jt].com;
+      double ex = cj.x - cx, ey = cj.y - cy, ez = cj.z - cz;
+      ex -= ra.L[0] * rint(ex * ra.Li[0]);
+      ey -= ra.L[1] * rint(ey * ra.Li[1]);
+      ez -= ra.L[2] * rint(ez * ra.Li[2]);
+      const double reach = ra.rc_pair + cj.w + rad + 1e-9;
+      cand = ex * ex + ey * ey + ez * ez <= reach * reach;
+    }
+    uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
+    while (mask) {
+      const int j = base + __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const FrameMol m = fmol[j];
+      if (uni)
+        site_pairs<P, true>(ra, m, fsite, ftype, sh, etab, cx, cy, cz, !dead, a, qa, px, py, pz, ev, eq, ecb,
+                            over);
+      else
+        site_pairs<P, false>(ra, m, fsite, ftype, sh, etab, cx, cy, cz, !dead, a, qa, px, py, pz, ev, eq, ecb,
+                             over);
+      dead = dead || over;
+      if (!wave_any(!dead)) break;
+    }
+  }
+  if (active) out[v] = make_double4(ev, eq, ecb, over ? 1.0 : 0.0);
+}
+
+// widom_var and the block sums of one insertion per lane (tail of the
+// insertion loop of Widom_Insert, widom_insert.f90:436-447).
+__device__ inline void widom_finish(const WidomArgs &w, bool active, int64_t i, double e_inter, double ecb,
+                                    double de_recip, bool over, double *part, int64_t *part_ov, double *wvar,
+                                    double *eout) {
+  const double dE = e_inter + de_recip + w.e_const;
+  // Build_Molecule marks a trial whose CBMC-cutoff energy reaches max_kBT as
+  // overlapped (fragment_growth.f90:536-539)
+  const bool cbmc_over = over || (w.beta * ecb >= kMaxKBT);
+  double wv = 0.0;
+  int64_t ovl = 0;
+  if (active) {
+    if (!cbmc_over) wv = exp(-w.beta * dE);
+    ovl = cbmc_over ? 1 : 0;
+    if (wvar) wvar[i] = wv;
+    if (eout) {
+      eout[4 * i] = e_inter;
+      eout[4 * i + 1] = ecb;
+      eout[4 * i + 2] = de_recip;
+      eout[4 * i + 3] = over ? 1.0 : 0.0;
+    }
+  }
+  __shared__ double shv[kWBlock / 64];
+  __shared__ int64_t sho[kWBlock / 64];
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    wv += __shfl_down(wv, d, 64);
+    ovl += __shfl_down(ovl, d, 64);
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) {
+    shv[wid] = wv;
+    sho[wid] = ovl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = 0.0;
+    int64_t c = 0;
+#pragma unroll
+    for (int k = 0; k < kWBlock / 64; ++k) {
+      a += shv[k];
+      c += sho[k];
+    }
+    part[blockIdx.x] = a;
+    part_ov[blockIdx.x] = c;
+  }
+}
+
+template <int NS>
+__global__ void __launch_bounds__(kWBlock) k_widom_combine(WidomArgs w, BoxDev b, EwaldDev ew,
+                                                           const double4 *__restrict__ wk,
+                                                           const double *__restrict__ pos,
+                                                           const double4 *__restrict__ sp, double *part,
+                                                           int64_t *part_ov, double *wvar, double *eout) {
+  const int64_t i0 = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  const bool active = i0 < w.n_ins;
+  const int64_t i = active ? i0 : w.n_ins - 1;
+  double ev = 0.0, eq = 0.0, ecb = 0.0;
+  bool over = false;
+  double px[NS], py[NS], pz[NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    const double4 t = sp[i * NS + a];
+    ev += t.x;
+    eq += t.y;
+    ecb += t.z;
+    over = over || t.w != 0.0;
+    px[a] = pos[(i * NS + a) * 3];
+    py[a] = pos[(i * NS + a) * 3 + 1];
+    pz[a] = pos[(i * NS + a) * 3 + 2];
+  }
+  double de_recip = 0.0;
+  if (w.charged && lanes_any(active && !over)) {
+    const double d = recip_delta_n<NS>(ew, wk, b, w, px, py, pz);
+    if (!over) de_recip = d;
+  }
+  widom_finish(w, active, i, ev + eq, ecb, de_recip, over, part, part_ov, wvar, eout);
+}
+
 template <typename P, int NS, bool GEN>
 __global__ void __launch_bounds__(kWBlock) k_widom_sorted(WidomArgs w, BoxDev b, WidomFrame f, WidomPairTab pt,
-                                                          EwaldDev ew, PairConst<double> cd, PairConst<float> cf,
+                                                          EwaldDev ew, const double4 *__restrict__ wk,
+                                                          PairConst<double> cd, PairConst<float> cf,
                                                           int coul_cbmc, const int *__restrict__ order,
                                                           double *__restrict__ part, int64_t *__restrict__ part_ov,
                                                           double *__restrict__ wvar, double *__restrict__ eout) {
   const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
   const bool active = s < w.n_ins;
   const int64_t i = order[active ? s : w.n_ins - 1];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
   double px[NS], py[NS], pz[NS];
   if (w.place) {
 #pragma unroll
@@ -380,57 +750,17 @@ __global__ void __launch_bounds__(kWBlock) k_widom_sorted(WidomArgs w, BoxDev b,
       if (!lanes_any(!dead)) break;
     }
   }
-  const double e_inter = ev + eq;
   double de_recip = 0.0;
   if (w.charged && lanes_any(!dead)) {
-    const double d = recip_delta_n<NS>(ew, b, w, px, py, pz);
+    const double d = recip_delta_n<NS>(ew, wk, b, w, px, py, pz);
     if (!over) de_recip = d;
   }
-  const double dE = e_inter + de_recip + w.e_const;
-  // Build_Molecule marks a trial whose CBMC-cutoff energy reaches max_kBT as
-  // overlapped (fragment_growth.f90:536-539)
-  const bool cbmc_over = over || (w.beta * ecb >= kMaxKBT);
-  double wv = 0.0;
-  int64_t ovl = 0;
-  if (active) {
-    if (!cbmc_over) wv = exp(-w.beta * dE);
-    ovl = cbmc_over ? 1 : 0;
-    if (wvar) wvar[i] = wv;
-    if (eout) {
-      eout[4 * i] = e_inter;
-      eout[4 * i + 1] = ecb;
-      eout[4 * i + 2] = de_recip;
-      eout[4 * i + 3] = over ? 1.0 : 0.0;
-    }
-  }
-  // block sums of widom_var and of the overlap count
-  __shared__ double shv[kWBlock / 64];
-  __shared__ int64_t sho[kWBlock / 64];
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    wv += __shfl_down(wv, d, 64);
-    ovl += __shfl_down(ovl, d, 64);
-  }
-  if (lane == 0) {
-    shv[wid] = wv;
-    sho[wid] = ovl;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double a = 0.0;
-    int64_t c = 0;
-#pragma unroll
-    for (int k = 0; k < kWBlock / 64; ++k) {
-      a += shv[k];
-      c += sho[k];
-    }
-    part[blockIdx.x] = a;
-    part_ov[blockIdx.x] = c;
-  }
+  widom_finish(w, active, i, ev + eq, ecb, de_recip, over, part, part_ov, wvar, eout);
 }
 
-typedef void (*widom_kernel_t)(WidomArgs, BoxDev, WidomFrame, WidomPairTab, EwaldDev, PairConst<double>,
-                               PairConst<float>, int, const int *, double *, int64_t *, double *, double *);
+typedef void (*widom_kernel_t)(WidomArgs, BoxDev, WidomFrame, WidomPairTab, EwaldDev, const double4 *,
+                               PairConst<double>, PairConst<float>, int, const int *, double *, int64_t *, double *,
+                               double *);
 
 template <typename P, bool GEN>
 static widom_kernel_t widom_kernel_for(int ns) {
@@ -447,13 +777,12 @@ static widom_kernel_t widom_kernel_for(int ns) {
   }
 }
 
-// GEN: every vdW and Coulomb style; otherwise the common ones only (LJ cut,
-// cut_tail, cut_shift; Ewald, cut Coulomb, none), whose pair code needs
-// fewer registers.
-static widom_kernel_t widom_kernel(int prec, int ns, bool gen) {
-  if (prec == CASS_PREC_MIXED) return gen ? widom_kernel_for<PrecMixed, true>(ns) : widom_kernel_for<PrecMixed, false>(ns);
-  if (prec == CASS_PREC_FP32) return gen ? widom_kernel_for<PrecFP32, true>(ns) : widom_kernel_for<PrecFP32, false>(ns);
-  return gen ? widom_kernel_for<PrecFP64, true>(ns) : widom_kernel_for<PrecFP64, false>(ns);
+// The fused kernel with every vdW and Coulomb style (the split path covers the
+// common ones).
+static widom_kernel_t widom_kernel(int prec, int ns) {
+  if (prec == CASS_PREC_MIXED) return widom_kernel_for<PrecMixed, true>(ns);
+  if (prec == CASS_PREC_FP32) return widom_kernel_for<PrecFP32, true>(ns);
+  return widom_kernel_for<PrecFP64, true>(ns);
 }
 
 }  // namespace cass
@@ -463,6 +792,8 @@ using namespace cass;
 BoxDev cass_box_view(const cass_engine *e, const cass_box_state &b);  // engine.hip
 EwaldDev cass_ewald_view(const cass_box_state &b);
 int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs &w);  // host.cpp
+
+static bool bv_cubic(const cass_box_state &b) { return b.p.box_shape == CASS_BOX_CUBIC; }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -476,37 +807,78 @@ static void widom_grid(const cass_box_state &b, int64_t n, int g[3]) {
   for (int d = 0; d < 3; ++d) g[d] = (int)std::min(64.0, std::max(1.0, std::floor(L[d] / side + 0.5)));
 }
 
+typedef void (*place_kernel_t)(WidomArgs, BoxDev, int, int, int, double *, unsigned *, int *);
+typedef void (*combine_kernel_t)(WidomArgs, BoxDev, EwaldDev, const double4 *, const double *, const double4 *,
+                                 double *, int64_t *, double *, double *);
+typedef void (*real_kernel_t)(RealArgs, const FrameMol *, const double4 *, const int *, const double *, const int *,
+                              const double4 *, const double *, const double *, double4 *);
+
+template <int NS>
+static void split_kernels(place_kernel_t &pl, combine_kernel_t &co) {
+  pl = k_widom_place<NS>;
+  co = k_widom_combine<NS>;
+}
+
+static bool split_kernels_for(int ns, place_kernel_t &pl, combine_kernel_t &co) {
+  switch (ns) {
+    case 1: split_kernels<1>(pl, co); return true;
+    case 2: split_kernels<2>(pl, co); return true;
+    case 3: split_kernels<3>(pl, co); return true;
+    case 4: split_kernels<4>(pl, co); return true;
+    case 5: split_kernels<5>(pl, co); return true;
+    case 6: split_kernels<6>(pl, co); return true;
+    case 7: split_kernels<7>(pl, co); return true;
+    case 8: split_kernels<8>(pl, co); return true;
+    default: return false;
+  }
+}
+
+static real_kernel_t real_kernel(int prec) {
+  if (prec == CASS_PREC_MIXED) return k_widom_real<PrecMixed>;
+  if (prec == CASS_PREC_FP32) return k_widom_real<PrecFP32>;
+  return k_widom_real<PrecFP64>;
+}
+
 int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed, const double *place,
                    double *widom_var, double *eout, cass_widom_result *res) {
-  cass_box_state *bp = nullptr;
   if (!e || box < 0 || box >= (int)e->boxes.size()) return CASS_ERR_ARG;
-  bp = &e->boxes[box];
-  cass_box_state &b = *bp;
-  if (species < 0 || species >= e->nspecies || n_ins <= 0 || n_ins > (int64_t)INT32_MAX) return CASS_ERR_ARG;
+  cass_box_state &b = e->boxes[box];
+  if (species < 0 || species >= e->nspecies || n_ins <= 0) return CASS_ERR_ARG;
   const int ns = e->sp_natoms[species];
   if (ns > kMaxWidomSites || ns < 1) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+  if (n_ins * ns > (int64_t)INT32_MAX) return cass_fail(e, CASS_ERR_ARG, "too many insertions in one batch");
   WidomArgs w;
   int rc = cass_widom_prepare(e, b, species, w);
   if (rc) return rc;
   w.n_ins = n_ins;
   w.seed = seed;
+  const cass_params &pp = b.p;
   // every pair contribution of these styles needs r <= rc_pair
   // (Check_AtomPair_Cutoff, energy_routines.f90:3092-3195)
   {
-    const cass_params &p = b.p;
-    const bool vdw_cut = p.vdw_style == CASS_VDW_NONE ||
-                         ((p.vdw_style == CASS_VDW_LJ || p.vdw_style == CASS_VDW_MIE) &&
-                          (p.vdw_sum_style == CASS_SUM_CUT || p.vdw_sum_style == CASS_SUM_CUT_TAIL ||
-                           p.vdw_sum_style == CASS_SUM_CUT_SHIFT || p.vdw_sum_style == CASS_SUM_CUT_SWITCH ||
-                           p.vdw_sum_style == CASS_SUM_CUT_SHIFT_FORCE));
-    const bool q_cut = p.charge_style == CASS_Q_NONE || p.charge_style == CASS_Q_EWALD ||
-                       p.charge_style == CASS_Q_DSF || p.charge_style == CASS_Q_CUT;
-    const double rc2 = std::max(std::max(p.rcut_vdwsq, p.rcut_coulsq),
-                                std::max(p.rcut_cbmcsq, p.vdw_sum_style == CASS_SUM_CUT_SWITCH ? p.roff_switch_sq : 0.0));
+    const bool vdw_cut = pp.vdw_style == CASS_VDW_NONE ||
+                         ((pp.vdw_style == CASS_VDW_LJ || pp.vdw_style == CASS_VDW_MIE) &&
+                          (pp.vdw_sum_style == CASS_SUM_CUT || pp.vdw_sum_style == CASS_SUM_CUT_TAIL ||
+                           pp.vdw_sum_style == CASS_SUM_CUT_SHIFT || pp.vdw_sum_style == CASS_SUM_CUT_SWITCH ||
+                           pp.vdw_sum_style == CASS_SUM_CUT_SHIFT_FORCE));
+    const bool q_cut = pp.charge_style == CASS_Q_NONE || pp.charge_style == CASS_Q_EWALD ||
+                       pp.charge_style == CASS_Q_DSF || pp.charge_style == CASS_Q_CUT;
+    const double rc2 =
+        std::max(std::max(pp.rcut_vdwsq, pp.rcut_coulsq),
+                 std::max(pp.rcut_cbmcsq, pp.vdw_sum_style == CASS_SUM_CUT_SWITCH ? pp.roff_switch_sq : 0.0));
     w.cutoff_only = (vdw_cut && q_cut) ? 1 : 0;
     w.rc_pair2 = rc2;
     w.rc_pair = std::sqrt(rc2);
   }
+  // the split path covers LJ cut / cut_tail / cut_shift with Ewald, cut
+  // Coulomb or no charges; the fused kernel handles every other style
+  const bool common = (pp.vdw_style == CASS_VDW_NONE ||
+                       (pp.vdw_style == CASS_VDW_LJ &&
+                        (pp.vdw_sum_style == CASS_SUM_CUT || pp.vdw_sum_style == CASS_SUM_CUT_TAIL ||
+                         pp.vdw_sum_style == CASS_SUM_CUT_SHIFT))) &&
+                      (pp.charge_style == CASS_Q_NONE || pp.charge_style == CASS_Q_EWALD ||
+                       pp.charge_style == CASS_Q_CUT) &&
+                      bv_cubic(b);
 
   // frame tables (host mirrors -> device)
   std::vector<FrameMol> fm;
@@ -528,42 +900,69 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
       ft.push_back(e->atom_type[off + a]);
     }
   }
-  const int nt = e->ntypes;
-  std::vector<double4> tab((size_t)ns * std::max(nt, 1));
+  const int nt = std::max(e->ntypes, 1);
+  std::vector<double4> tab((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
+  // split-path LJ table: (4 eps, sigma^2, shift, 0); the shift is the
+  // cut_shift constant 4 eps (c^12 - c^6), c^2 = sigma^2 / rc_vdw^2
+  std::vector<double4> tab2((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
   for (int a = 0; a < ns; ++a)
-    for (int t = 0; t < nt; ++t) {
+    for (int t = 0; t < e->ntypes; ++t) {
       const int ti = w.type[a];
-      if (ti < 0) {
-        tab[(size_t)a * nt + t] = make_double4(0.0, 0.0, 0.0, 0.0);
-        continue;
-      }
-      const size_t k = (size_t)ti * nt + t;
+      if (ti < 0 || pp.vdw_style == CASS_VDW_NONE) continue;
+      const size_t k = (size_t)ti * e->ntypes + t;
       tab[(size_t)a * nt + t] = make_double4(e->eps[k], e->sig[k], e->mie_n[k], e->mie_m[k]);
+      const double eps = e->eps[k], sig = e->sig[k];
+      double shift = 0.0;
+      if (pp.vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+        const double c2 = sig * sig / pp.rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        shift = 4.0 * eps * (c12 - c6);
+      }
+      tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
     }
 
-  const int nblk = (int)((n_ins + kWBlock - 1) / kWBlock);
+  // exp(-k/32), k = 0 .. nexp-1, covering -(alpha rc_pair)^2 (at most 4096)
+  std::vector<double> etab;
+  if (common && pp.charge_style == CASS_Q_EWALD) {
+    const double xm = pp.alpha_ewald * w.rc_pair;
+    const int ne = (int)std::min(4096.0, std::ceil(32.0 * xm * xm) + 2.0);
+    etab.resize(ne);
+    for (int k = 0; k < ne; ++k) etab[k] = std::exp(-k / 32.0);
+  }
+  const size_t n = (size_t)n_ins;
+  const size_t nslot = common ? n * ns : n;
+  const int nblk = (int)((n + kWBlock - 1) / kWBlock);
   int g[3];
-  widom_grid(b, n_ins, g);
+  widom_grid(b, (int64_t)n, g);  // about 64 sites of one kind per cell
   const int ncell = g[0] * g[1] * g[2];
   int bits = 1;
-  while ((1 << bits) < ncell) ++bits;
+  while ((1ll << bits) < (int64_t)ncell * (common ? ns : 1)) ++bits;
   size_t sort_tmp = 0;
   HIP_TRY(e, rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned *)nullptr, (unsigned *)nullptr, (int *)nullptr,
-                                       (int *)nullptr, (unsigned)n_ins, 0, bits, e->stream));
-  const size_t n = (size_t)n_ins;
+                                       (int *)nullptr, (unsigned)nslot, 0, bits, e->stream));
   size_t off = 0;
-  const size_t o_part = off; off = align256(off + sizeof(double) * nblk);
-  const size_t o_pov = off; off = align256(off + sizeof(int64_t) * nblk);
-  const size_t o_sum = off; off = align256(off + 64);
-  const size_t o_k0 = off; off = align256(off + sizeof(unsigned) * n);
-  const size_t o_k1 = off; off = align256(off + sizeof(unsigned) * n);
-  const size_t o_v0 = off; off = align256(off + sizeof(int) * n);
-  const size_t o_v1 = off; off = align256(off + sizeof(int) * n);
-  const size_t o_fm = off; off = align256(off + sizeof(FrameMol) * std::max<size_t>(fm.size(), 1));
-  const size_t o_fs = off; off = align256(off + sizeof(double4) * std::max<size_t>(fs.size(), 1));
-  const size_t o_ft = off; off = align256(off + sizeof(int) * std::max<size_t>(ft.size(), 1));
-  const size_t o_tab = off; off = align256(off + sizeof(double4) * tab.size());
-  const size_t o_tmp = off; off = align256(off + sort_tmp);
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = align256(off + std::max<size_t>(bytes, 1));
+    return o;
+  };
+  const size_t o_part = carve(sizeof(double) * nblk);
+  const size_t o_pov = carve(sizeof(int64_t) * nblk);
+  const size_t o_sum = carve(64);
+  const size_t o_k0 = carve(sizeof(unsigned) * nslot);
+  const size_t o_k1 = carve(sizeof(unsigned) * nslot);
+  const size_t o_v0 = carve(sizeof(int) * nslot);
+  const size_t o_v1 = carve(sizeof(int) * nslot);
+  const size_t o_fm = carve(sizeof(FrameMol) * fm.size());
+  const size_t o_fs = carve(sizeof(double4) * fs.size());
+  const size_t o_ft = carve(sizeof(int) * ft.size());
+  const size_t o_tab = carve(sizeof(double4) * tab.size());
+  const size_t o_tab2 = carve(sizeof(double4) * tab2.size());
+  const size_t o_etab = carve(sizeof(double) * etab.size());
+  const size_t o_q = carve(sizeof(double) * kMaxWidomSites);
+  const size_t o_pos = carve(common && !place ? sizeof(double) * 3 * n * ns : 0);
+  const size_t o_sp = carve(common ? sizeof(double4) * n * ns : 0);
+  const size_t o_wk = carve(sizeof(double4) * std::max(b.nvecs, 1));
+  const size_t o_tmp = carve(sort_tmp);
   char *s = (char *)cass_scratch(e, off);
   if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
   double *part = (double *)(s + o_part);
@@ -576,12 +975,22 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   double4 *d_fs = (double4 *)(s + o_fs);
   int *d_ft = (int *)(s + o_ft);
   double4 *d_tab = (double4 *)(s + o_tab);
+  double4 *d_tab2 = (double4 *)(s + o_tab2);
+  double *d_etab = (double *)(s + o_etab);
+  double *d_q = (double *)(s + o_q);
+  double *d_pos = (double *)(s + o_pos);
+  double4 *d_sp = (double4 *)(s + o_sp);
+  double4 *d_wk = (double4 *)(s + o_wk);
   if (!fm.empty()) {
     HIP_TRY(e, hipMemcpyAsync(d_fm, fm.data(), sizeof(FrameMol) * fm.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_fs, fs.data(), sizeof(double4) * fs.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_ft, ft.data(), sizeof(int) * ft.size(), hipMemcpyHostToDevice, e->stream));
   }
   HIP_TRY(e, hipMemcpyAsync(d_tab, tab.data(), sizeof(double4) * tab.size(), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(d_tab2, tab2.data(), sizeof(double4) * tab2.size(), hipMemcpyHostToDevice, e->stream));
+  if (!etab.empty())
+    HIP_TRY(e, hipMemcpyAsync(d_etab, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(d_q, w.q, sizeof(double) * kMaxWidomSites, hipMemcpyHostToDevice, e->stream));
 
   double *d_place = nullptr, *d_var = nullptr, *d_eout = nullptr;
   if (place) {
@@ -601,28 +1010,72 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   PairConst<double> cd = make_pair_const<double>(b.p, e->ntypes);
   PairConst<float> cf = make_pair_const<float>(b.p, e->ntypes);
   WidomFrame wf{d_fm, d_fs, d_ft, (int)fm.size()};
-  WidomPairTab pt{d_tab, nt};
   const int coul_cbmc = e->sp_coul_cbmc[species];
 
-  // spatial order of the insertions
-  hipLaunchKernelGGL(k_widom_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, w, bv, g[0], g[1],
-                     g[2], k0, v0);
-  HIP_TRY(e, hipGetLastError());
-  HIP_TRY(e, rocprim::radix_sort_pairs((void *)(s + o_tmp), sort_tmp, k0, k1, v0, v1, (unsigned)n, 0, bits,
-                                       e->stream));
-  const cass_params &pp = b.p;
-  const bool common = (pp.vdw_style == CASS_VDW_NONE ||
-                       (pp.vdw_style == CASS_VDW_LJ &&
-                        (pp.vdw_sum_style == CASS_SUM_CUT || pp.vdw_sum_style == CASS_SUM_CUT_TAIL ||
-                         pp.vdw_sum_style == CASS_SUM_CUT_SHIFT))) &&
-                      (pp.charge_style == CASS_Q_NONE || pp.charge_style == CASS_Q_EWALD ||
-                       pp.charge_style == CASS_Q_CUT);
-  widom_kernel_t kern = widom_kernel(e->prec, ns, !common);
-  HIP_TRY(e, hipEventRecord(e->ev0, e->stream));
-  hipLaunchKernelGGL(kern, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, wf, pt, ew, cd, cf, coul_cbmc, v1, part,
-                     part_ov, d_var, d_eout);
-  HIP_TRY(e, hipGetLastError());
-  HIP_TRY(e, hipEventRecord(e->ev1, e->stream));
+  HIP_TRY(e, hipEventRecord(e->evs[0], e->stream));
+  if (w.charged && ew.nvecs > 0) {
+    hipLaunchKernelGGL(k_widom_weights, dim3((ew.nvecs + 255) / 256), dim3(256), 0, e->stream, ew, d_wk);
+    HIP_TRY(e, hipGetLastError());
+  }
+  if (common) {
+    place_kernel_t pl;
+    combine_kernel_t co;
+    if (!split_kernels_for(ns, pl, co)) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+    const double *pos = place ? d_place : d_pos;
+    hipLaunchKernelGGL(pl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, w, bv, g[0], g[1], g[2],
+                       d_pos, k0, v0);
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, rocprim::radix_sort_pairs((void *)(s + o_tmp), sort_tmp, k0, k1, v0, v1, (unsigned)nslot, 0, bits,
+                                         e->stream));
+    HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
+    const unsigned rblk = (unsigned)((nslot + kWBlock - 1) / kWBlock);
+    RealArgs ra;
+    std::memset(&ra, 0, sizeof(ra));
+    ra.nslot = (int64_t)nslot;
+    ra.ns = ns;
+    ra.ntypes = nt;
+    ra.nmols = (int)fm.size();
+    ra.q_style = pp.charge_style;
+    ra.coul_cbmc = coul_cbmc;
+    ra.nexp = (int)etab.size();
+    ra.rc_vdw2 = pp.vdw_style == CASS_VDW_NONE ? -1.0 : pp.rcut_vdwsq;
+    ra.rc_coul2 = pp.charge_style == CASS_Q_NONE ? -1.0 : pp.rcut_coulsq;
+    ra.rc_cbmc2 = pp.rcut_cbmcsq;
+    ra.rc_low2 = pp.rcut_lowsq;
+    ra.rc_pair2 = w.rc_pair2;
+    ra.rc_pair = w.rc_pair;
+    ra.alpha = pp.alpha_ewald;
+    for (const FrameMol &r : fm) ra.dcom_max = std::max(ra.dcom_max, r.com.w);
+    ra.same_cut = (pp.vdw_style == CASS_VDW_NONE || pp.rcut_vdwsq == w.rc_pair2) &&
+                  (pp.charge_style == CASS_Q_NONE || pp.rcut_coulsq == w.rc_pair2);
+    for (int d = 0; d < 3; ++d) {
+      ra.L[d] = pp.basis_length[d];
+      ra.hL[d] = pp.hlength[d];
+      ra.Li[d] = 1.0 / pp.basis_length[d];
+    }
+    const size_t shbytes = sizeof(double4) * tab2.size() + sizeof(double) * etab.size();
+    hipLaunchKernelGGL(real_kernel(e->prec), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_ft,
+                       pos, v1, d_tab2, d_q, d_etab, d_sp);
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
+    hipLaunchKernelGGL(co, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, ew, d_wk, pos, d_sp, part, part_ov,
+                       d_var, d_eout);
+    HIP_TRY(e, hipGetLastError());
+  } else {
+    WidomPairTab pt{d_tab, nt};
+    hipLaunchKernelGGL(k_widom_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, w, bv, g[0], g[1],
+                       g[2], k0, v0);
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, rocprim::radix_sort_pairs((void *)(s + o_tmp), sort_tmp, k0, k1, v0, v1, (unsigned)nslot, 0, bits,
+                                         e->stream));
+    HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
+    widom_kernel_t kern = widom_kernel(e->prec, ns);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, wf, pt, ew, d_wk, cd, cf, coul_cbmc,
+                       v1, part, part_ov, d_var, d_eout);
+    HIP_TRY(e, hipGetLastError());
+    HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
+  }
+  HIP_TRY(e, hipEventRecord(e->evs[3], e->stream));
   hipLaunchKernelGGL(k_sum_widom, dim3(1), dim3(256), 0, e->stream, part, part_ov, nblk, d_sum, d_nov);
   HIP_TRY(e, hipGetLastError());
   double hs[2];
@@ -633,9 +1086,14 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   if (d_var) HIP_TRY(e, hipFreeAsync(d_var, e->stream));
   if (d_eout) HIP_TRY(e, hipFreeAsync(d_eout, e->stream));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
-  float ms = 0.0f;
-  HIP_TRY(e, hipEventElapsedTime(&ms, e->ev0, e->ev1));
-  e->last_kernel_ms = ms;
+  float ms[3] = {0.0f, 0.0f, 0.0f}, tot = 0.0f;
+  HIP_TRY(e, hipEventElapsedTime(&ms[0], e->evs[0], e->evs[1]));
+  HIP_TRY(e, hipEventElapsedTime(&ms[1], e->evs[1], e->evs[2]));
+  HIP_TRY(e, hipEventElapsedTime(&ms[2], e->evs[2], e->evs[3]));
+  HIP_TRY(e, hipEventElapsedTime(&tot, e->evs[0], e->evs[3]));
+  for (int k = 0; k < 3; ++k) e->last_stage_ms[k] = ms[k];
+  e->last_stage_ms[3] = tot;
+  e->last_kernel_ms = ms[1] + ms[2];
   res->widom_sum = hs[0];
   std::memcpy(&res->n_overlaps, &hs[1], sizeof(int64_t));
   res->n_insertions = n_ins;

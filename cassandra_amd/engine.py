@@ -54,6 +54,7 @@ def load_library(path=LIB_PATH):
         "cass_set_precision": (i32, [vp, i32]),
         "cass_last_error": (C.c_char_p, [vp]),
         "cass_last_kernel_ms": (i32, [vp, P(d)]),
+        "cass_last_stage_ms": (i32, [vp, P(d)]),
         "cass_box_create": (i32, [vp, P(abi.CassParams), i32, P(i32)]),
         "cass_box_upload": (i32, [vp, i32, i32, P(i32), P(i32), P(d), P(d), P(d)]),
         "cass_mol_update": (i32, [vp, i32, i32, i32, P(d), P(d), P(d)]),
@@ -157,6 +158,13 @@ class Engine:
         ms = C.c_double()
         self._chk(self.lib.cass_last_kernel_ms(self.h, C.byref(ms)), "cass_last_kernel_ms")
         return ms.value
+
+    def last_stage_ms(self):
+        """(placement + sort, real space, reciprocal + combine, whole batch) of
+        the last Widom batch, in ms of device time."""
+        ms = (C.c_double * 4)()
+        self._chk(self.lib.cass_last_stage_ms(self.h, ms), "cass_last_stage_ms")
+        return list(ms)
 
     def set_precision(self, prec):
         self._chk(self.lib.cass_set_precision(self.h, prec), "cass_set_precision")

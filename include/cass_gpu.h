@@ -144,6 +144,11 @@ const char *cass_last_error(cass_engine *eng);
  * counterpart in the reference (whose timers are the CPU clocks of
  * nvtmc_driver.f90:110-232 and widom_insert.f90:322-323). */
 int cass_last_kernel_ms(cass_engine *eng, double *ms);
+/* Stage times of the last cass_widom_batch / cass_widom_placements, from the
+ * same events: ms[0] placement and spatial sort, ms[1] real-space pair
+ * kernel, ms[2] reciprocal change and combination, ms[3] the whole batch on
+ * the device.  Instrumentation; no counterpart in the reference. */
+int cass_last_stage_ms(cass_engine *eng, double *ms);
 
 /* Adds a box able to hold `max_mols` molecules; returns its id in *box. */
 int cass_box_create(cass_engine *eng, const cass_params *p, int32_t max_mols, int32_t *box);
