@@ -38,8 +38,11 @@ def widom_frame():
 
 
 def pair_counts(system, sp_index, n_sample, seed):
-    """Average number of LJ and q-q site pairs inside the cutoff per insertion
-    of the probe, over a sample of uniform random insertions."""
+    """Per attempted insertion (uniform random placements): the LJ and q-q
+    site pairs inside the cutoff that the algorithm must evaluate, and the
+    fraction of insertions that survive the hard-overlap test.  An insertion
+    with a site pair closer than rcut_low is rejected on that pair, as the
+    reference's Build_Molecule does, so its pairs are not counted."""
     from cassandra_amd import systems as S
     probe = system.species[sp_index]
     rng = np.random.default_rng(seed)
@@ -51,19 +54,27 @@ def pair_counts(system, sp_index, n_sample, seed):
                          for sp, n in zip(system.species, system.nmols)])
     ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
     rc2 = system.rcut_vdw ** 2
+    rlow2 = system.rcut_low ** 2
     n_lj = n_q = 0
+    n_live = 0
     for _ in range(n_sample):
         c = (rng.random(3) - 0.5) * L
         xyz = ref @ S.random_rotation(rng).T + c
-        for a, at in enumerate(probe.atoms):
+        r2s = []
+        for a in range(probe.natoms):
             d = frame - xyz[a]
             d -= L * np.round(d / L)
-            inr = (d * d).sum(1) <= rc2
+            r2s.append((d * d).sum(1))
+        if min(r2.min() for r2 in r2s) < rlow2:
+            continue
+        n_live += 1
+        for a, at in enumerate(probe.atoms):
+            inr = r2s[a] <= rc2
             if at.eps_K > 0:
                 n_lj += int((inr & lj).sum())
             if at.charge != 0:
                 n_q += int((inr & (q != 0)).sum())
-    return n_lj / n_sample, n_q / n_sample
+    return n_lj / n_sample, n_q / n_sample, n_live / n_sample
 
 
 # Algorithmic flops per unit (an FMA counts 2; sqrt, division and the exp of
@@ -78,9 +89,11 @@ def pair_counts(system, sp_index, n_sample, seed):
 FL_GEOM, FL_LJ, FL_EWALD, FL_KSITE, FL_K = 11.0, 8.0, 23.0, 10.0, 10.0
 
 
-def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K):
-    """Per insertion: {"real": (FP32, FP64), "recip": (FP32, FP64)} flops of
-    the real-space pair kernel and of the reciprocal/combine kernel.  Pair
+def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K, f_live=1.0):
+    """Per attempted insertion: {"real": (FP32, FP64), "recip": (FP32, FP64)}
+    flops of the real-space pair kernel and of the reciprocal kernel (only
+    the fraction f_live of insertions that survive the overlap test needs
+    the reciprocal change).  Pair
     geometry and the reciprocal change are FP64 in every mode; the pair terms
     are FP32 in the fp32 mode only (mixed refines FP32 hardware seeds in
     FP64)."""
@@ -88,7 +101,7 @@ def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K):
     pair = FL_LJ * n_lj + FL_EWALD * n_q
     geom = FL_GEOM * max(n_lj, n_q)
     real = (pair, geom) if prec == E.FP32 else (0.0, pair + geom)
-    recip = (0.0, (FL_KSITE * n_probe_sites + FL_K) * K)
+    recip = (0.0, f_live * (FL_KSITE * n_probe_sites + FL_K) * K)
     return {"real": real, "recip": recip}
 
 
@@ -224,7 +237,7 @@ def main():
     eng.set_precision(E.MIXED)
     dt_mixed, k_mixed = time_widom(box, sp, n, 1234, a.steps, a.warmup)
     value = n * a.steps / dt_mixed
-    log(f"mixed: {value:.4g} trial energies/s, stages (place+sort, real, recip+combine, batch) "
+    log(f"mixed: {value:.4g} trial energies/s, stages (place+sort+overlap, real, recip+finish, batch) "
         f"{', '.join(f'{x:.3f}' for x in k_mixed)} ms")
     if a.profile_mixed:
         print(json.dumps({"value": value, "unit": "trial energies/s", "stage_ms": k_mixed, "steps": a.steps,
@@ -257,7 +270,7 @@ def main():
                       "overlap_flags_equal": bool(np.array_equal(w32 > 0, w64 > 0))}
     log(f"dE errors {errs}")
 
-    n_lj, n_q = pair_counts(system, sp, 200, 5)
+    n_lj, n_q, f_live = pair_counts(system, sp, 400, 5)
     n_charged = sum(1 for at in probe.atoms if at.charge != 0)
 
     def roof1(f32, f64, ms, label):
@@ -272,22 +285,23 @@ def main():
     def roof(prec, stages, tag):
         """Roofline of the dominant kernel of the batch (by device time), the
         other kernel and the whole batch."""
-        fl = algorithmic_flops(prec, n_lj, n_q, n_charged, K)
+        fl = algorithmic_flops(prec, n_lj, n_q, n_charged, K, f_live)
         r_real = roof1(*fl["real"], stages[1], f"k_widom_real<{tag}>")
-        r_recip = roof1(*fl["recip"], stages[2], "k_widom_combine<3>")
+        r_recip = roof1(*fl["recip"], stages[2], "k_widom_recip<3> + k_widom_finish<3>")
         tot32 = fl["real"][0] + fl["recip"][0]
         tot64 = fl["real"][1] + fl["recip"][1]
-        r_all = roof1(tot32, tot64, stages[3], "whole batch (place, sort, real, combine)")
+        r_all = roof1(tot32, tot64, stages[3], "whole batch (place, sort, overlap, select, real, recip, finish)")
         dom, other = (r_real, r_recip) if stages[1] >= stages[2] else (r_recip, r_real)
         dom = dict(dom)
         dom["other_kernel"] = other
         dom["batch"] = r_all
-        dom["stage_ms"] = {"place_sort": stages[0], "real": stages[1], "recip_combine": stages[2],
+        dom["stage_ms"] = {"place_sort_overlap_select": stages[0], "real": stages[1], "recip_finish": stages[2],
                            "batch": stages[3]}
         return dom
 
     roofline = roof(E.MIXED, k_mixed, "PrecMixed")
-    roofline["pairs_in_cutoff"] = {"lj": n_lj, "qq": n_q}
+    roofline["pairs_in_cutoff_per_insertion"] = {"lj": n_lj, "qq": n_q}
+    roofline["surviving_fraction"] = f_live
     roofline["kvectors"] = K
 
     extra = {}

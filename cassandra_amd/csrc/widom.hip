@@ -14,6 +14,8 @@
 // Results are written by insertion index, so the sort changes no value.
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -291,23 +293,28 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
 }
 
 // ------------------------------------------------------------ split path
-// For the common styles the insertion energy is formed in three kernels:
+// For the common styles the insertion energies are formed by:
 //  k_widom_place: sites of every insertion (counter stream or explicit) and a
 //    sort key per site = (site index in the molecule, cell of the site);
-//  k_widom_real: one lane per site, in sorted order, so the 64 lanes of a
-//    wavefront hold sites of one kind within one small cell: the pair code
-//    runs where some lane has the pair within the cutoff, and that region is
-//    barely wider than each lane's own;
-//  k_widom_combine: one lane per insertion: sums its sites' partials in site
+//  k_widom_overlap: the hard-overlap test of every site (one lane per site,
+//    sorted order), which rejects about half the insertions of a dense
+//    liquid before any energy is formed;
+//  k_widom_real: one lane per site of the surviving insertions, in sorted
+//    order, so the 64 lanes of a wavefront hold sites of one kind within one
+//    small cell: the pair code runs where some lane has the pair within the
+//    cutoff, and that region is barely wider than each lane's own;
+//  k_widom_recip: the reciprocal change of each surviving insertion;
+//  k_widom_finish: one lane per insertion: sums its sites' partials in site
 //    order, adds the reciprocal change and forms widom_var.
-// The site partials are (E_vdw, E_qq, E_cbmc, overlap) at [insertion][site].
+// The site partials are (E_vdw, E_qq, E_cbmc, 0) at [insertion][site].
 
 template <int NS>
 __global__ void __launch_bounds__(256) k_widom_place(WidomArgs w, BoxDev b, int g0, int g1, int g2,
                                                      double *__restrict__ pos, unsigned *__restrict__ keys,
-                                                     int *__restrict__ vals) {
+                                                     int *__restrict__ vals, int *__restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= w.n_ins) return;
+  flag[i] = 0;
   double px[NS], py[NS], pz[NS];
   if (w.place) {
 #pragma unroll
@@ -344,7 +351,7 @@ __global__ void __launch_bounds__(256) k_widom_place(WidomArgs w, BoxDev b, int 
 
 // Arguments of the real-space site kernel (cubic boxes, common styles).
 struct RealArgs {
-  int64_t nslot;
+  int64_t n_ins, npad;  // insertions; slots per site kind (multiple of 64)
   int ns, ntypes, nmols, q_style;  // q_style: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT
   int coul_cbmc, nexp, same_cut;   // same_cut: rc_vdw = rc_coul = rc_pair
   double rc_vdw2, rc_coul2, rc_cbmc2, rc_low2, rc_pair2, rc_pair, alpha;
@@ -446,20 +453,19 @@ __device__ inline void site_pair_energy(const RealArgs &ra, double r2, double4 p
   }
 }
 
-// Pair terms of one site (lane) against frame molecule m.  UNI: every site of
-// m takes the image shift of m's centre seen from the wave centre c,
-// S = -L rint((c - com)/L).  When (wave extent + cutoff + max_dcom) is below
-// half the box, S is the per-pair minimum image of
-// minimum_image_separation.f90:35-60 for every pair inside the cutoff, and
-// any other pair stays outside the cutoff; (p - s) + S is then the
-// reference's own separation, bit for bit.
+// Pair terms of one site (lane) against frame molecule m.  The wave holds
+// sites of one kind (a, qa, the LJ row are uniform), so the choice of terms
+// per frame site is a scalar branch.  UNI: every site of m takes the image
+// shift of m's centre seen from the wave centre c, S = -L rint((c - com)/L).
+// When (wave extent + cutoff + max_dcom) is below half the box, S is the
+// per-pair minimum image of minimum_image_separation.f90:35-60 for every
+// pair inside the cutoff, and any other pair stays outside the cutoff;
+// (p - s) + S is then the reference's own separation, bit for bit.
 template <typename P, bool UNI>
 __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
-                                  const int *__restrict__ ftype, const double4 *tab, const double *etab, double cx,
-                                  double cy, double cz, bool live, int a, double qa, double px, double py, double pz,
-                                  double &ev, double &eq, double &ecb, bool &over) {
-  // the shift of the molecule's centre serves all its sites (see the UNI
-  // condition in k_widom_real)
+                                  const int *__restrict__ ftype, const double4 *__restrict__ lj_row,
+                                  const double *etab, double cx, double cy, double cz, bool live, double qa,
+                                  double px, double py, double pz, double &ev, double &eq, double &ecb, bool &over) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (UNI) {
     sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
@@ -482,16 +488,17 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
     const double r2 = dx * dx + dy * dy + dz * dz;
     const bool near = live && r2 <= ra.rc_pair2;
     if (!wave_any(near)) continue;
+    // uniform per frame site: the LJ row entry and the charge product
+    const double4 prm = tj >= 0 ? lj_row[tj] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const double qq = qa * st.w;
+    const bool has_v = prm.x != 0.0;  // eps = 0 (or no type) adds exactly 0
+    const bool has_q = ra.q_style != CASS_Q_NONE && qq != 0.0;
     if (near) {
       over = over || r2 < ra.rc_low2;
       const bool incb = r2 <= ra.rc_cbmc2;
-      const double4 prm = tj >= 0 ? tab[a * ra.ntypes + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
-      const double qq = qa * st.w;
       const bool gv = r2 <= ra.rc_vdw2, gq = r2 <= ra.rc_coul2;
-      const bool do_v = prm.x != 0.0 && (gv || incb);  // eps = 0 (or no type) adds exactly 0
-      const bool do_q = ra.q_style != CASS_Q_NONE && qq != 0.0 && (gq || incb);
       double e_v = 0.0, e_q = 0.0;
-      site_pair_energy<P>(ra, r2, prm, qq, etab, do_v, do_q, e_v, e_q);
+      site_pair_energy<P>(ra, r2, prm, qq, etab, has_v, has_q, e_v, e_q);
       if (ra.same_cut) {
         ev += e_v;
         eq += e_q;
@@ -504,24 +511,33 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
   }
 }
 
+// One lane per site of the surviving insertions.  Slots are laid out kind by
+// kind, npad (a multiple of 64) per kind, so every wavefront holds sites of a
+// single kind.
 template <typename P>
 __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const FrameMol *__restrict__ fmol,
                                                         const double4 *__restrict__ fsite,
                                                         const int *__restrict__ ftype, const double *__restrict__ pos,
                                                         const int *__restrict__ order, const double4 *__restrict__ ptab,
                                                         const double *__restrict__ site_q,
-                                                        const double *__restrict__ etab_g, double4 *__restrict__ out) {
-  extern __shared__ double4 sh[];  // [site][type] (4 eps, sigma^2, shift, 0), then the exp table
-  const int ntab = ra.ns * ra.ntypes;
-  double *etab = (double *)(sh + ntab);
-  for (int t = threadIdx.x; t < ntab; t += blockDim.x) sh[t] = ptab[t];
+                                                        const double *__restrict__ etab_g,
+                                                        const int *__restrict__ nlive_sites,
+                                                        double4 *__restrict__ out) {
+  // surviving insertions: m per kind, npad = m rounded up to whole waves
+  const int64_t m = *nlive_sites / ra.ns;
+  const int64_t npad = (m + 63) / 64 * 64;
+  if ((int64_t)blockIdx.x * kWBlock >= (int64_t)ra.ns * npad) return;
+  extern __shared__ double etab[];  // exp(-k/32)
   for (int t = threadIdx.x; t < ra.nexp; t += blockDim.x) etab[t] = etab_g[t];
   __syncthreads();
   const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
-  const bool active = s < ra.nslot;
-  const int v = order[active ? s : ra.nslot - 1];
-  const int a = v % ra.ns;
+  const int a = __builtin_amdgcn_readfirstlane((int)min(s / npad, (int64_t)ra.ns - 1));
+  const int64_t idx = s - (int64_t)a * npad;
+  const bool active = s < (int64_t)ra.ns * npad && idx < m;
+  if (!wave_any(active)) return;
+  const int v = order[(int64_t)a * m + (active ? idx : m - 1)];
   const double qa = site_q[a];
+  const double4 *lj_row = ptab + (int64_t)a * ra.ntypes;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
   double ev = 0.0, eq = 0.0, ecb = 0.0;
   bool over = false;
@@ -553,10 +569,9 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
       mask &= mask - 1;
       const FrameMol m = fmol[j];
       if (uni)
-        site_pairs<P, true>(ra, m, fsite, ftype, sh, etab, cx, cy, cz, !dead, a, qa, px, py, pz, ev, eq, ecb,
-                            over);
+        site_pairs<P, true>(ra, m, fsite, ftype, lj_row, etab, cx, cy, cz, !dead, qa, px, py, pz, ev, eq, ecb, over);
       else
-        site_pairs<P, false>(ra, m, fsite, ftype, sh, etab, cx, cy, cz, !dead, a, qa, px, py, pz, ev, eq, ecb,
+        site_pairs<P, false>(ra, m, fsite, ftype, lj_row, etab, cx, cy, cz, !dead, qa, px, py, pz, ev, eq, ecb,
                              over);
       dead = dead || over;
       if (!wave_any(!dead)) break;
@@ -613,33 +628,120 @@ __device__ inline void widom_finish(const WidomArgs &w, bool active, int64_t i, 
   }
 }
 
-template <int NS>
-__global__ void __launch_bounds__(kWBlock) k_widom_combine(WidomArgs w, BoxDev b, EwaldDev ew,
-                                                           const double4 *__restrict__ wk,
+// Hard-overlap pre-pass: one lane per site (all insertions, kind-aligned
+// slots as in k_widom_real); flag[i] = 1 when a site of insertion i lies
+// within rcut_low of a frame site (the r^2 < rcut_low^2 test of
+// Compute_MoleculePair_Energy, energy_routines.f90:1676).  Such an insertion
+// is rejected without its energy, as Build_Molecule returns on the first
+// overlap (fragment_growth.f90:519-544); the energy kernels then run on the
+// surviving insertions only.  The candidate reach is rcut_low + max_dcom +
+// the wave radius, a handful of molecules.
+__global__ void __launch_bounds__(kWBlock) k_widom_overlap(RealArgs ra, const FrameMol *__restrict__ fmol,
+                                                           const double4 *__restrict__ fsite,
                                                            const double *__restrict__ pos,
-                                                           const double4 *__restrict__ sp, double *part,
-                                                           int64_t *part_ov, double *wvar, double *eout) {
-  const int64_t i0 = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
-  const bool active = i0 < w.n_ins;
-  const int64_t i = active ? i0 : w.n_ins - 1;
-  double ev = 0.0, eq = 0.0, ecb = 0.0;
+                                                           const int *__restrict__ order, int *__restrict__ flag) {
+  const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  const int a = __builtin_amdgcn_readfirstlane((int)min(s / ra.npad, (int64_t)ra.ns - 1));
+  const int64_t idx = s - (int64_t)a * ra.npad;
+  const bool active = s < (int64_t)ra.ns * ra.npad && idx < ra.n_ins;
+  if (!wave_any(active)) return;
+  const int v = order[(int64_t)a * ra.n_ins + (active ? idx : ra.n_ins - 1)];
+  const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
+  const double cx = wave_uniform(px), cy = wave_uniform(py), cz = wave_uniform(pz);
+  double dx = px - cx, dy = py - cy, dz = pz - cz;
+  dx -= ra.L[0] * rint(dx * ra.Li[0]);
+  dy -= ra.L[1] * rint(dy * ra.Li[1]);
+  dz -= ra.L[2] * rint(dz * ra.Li[2]);
+  const double rad = sqrt(wave_max(dx * dx + dy * dy + dz * dz));
+  const double rlow = sqrt(ra.rc_low2);
   bool over = false;
+  for (int base = 0; base < ra.nmols; base += 64) {
+    const int jt = base + (int)(threadIdx.x & 63);
+    bool cand = false;
+    if (jt < ra.nmols) {
+      const double4 cj = fmol[jt].com;
+      double ex = cj.x - cx, ey = cj.y - cy, ez = cj.z - cz;
+      ex -= ra.L[0] * rint(ex * ra.Li[0]);
+      ey -= ra.L[1] * rint(ey * ra.Li[1]);
+      ez -= ra.L[2] * rint(ez * ra.Li[2]);
+      const double reach = rlow + cj.w + rad + 1e-9;
+      cand = ex * ex + ey * ey + ez * ez <= reach * reach;
+    }
+    uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
+    while (mask) {
+      const int j = base + __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const FrameMol m = fmol[j];
+      for (int k = m.first; k < m.first + m.n; ++k) {
+        const double4 st = fsite[k];
+        double ex = px - st.x, ey = py - st.y, ez = pz - st.z;
+        // per-pair minimum image (rint form; equal to the cubic if/else rule
+        // for every separation shorter than half the box)
+        ex = fma(-rint(ex * ra.Li[0]), ra.L[0], ex);
+        ey = fma(-rint(ey * ra.Li[1]), ra.L[1], ey);
+        ez = fma(-rint(ez * ra.Li[2]), ra.L[2], ez);
+        over = over || ex * ex + ey * ey + ez * ez < ra.rc_low2;
+      }
+    }
+  }
+  if (active && over) flag[v / ra.ns] = 1;
+}
+
+// Selection predicates of the surviving sites and insertions.
+struct LiveSite {
+  const int *flag;
+  int ns;
+  __device__ bool operator()(int v) const { return flag[v / ns] == 0; }
+};
+struct LiveInsertion {
+  const int *flag;
+  __device__ bool operator()(int i) const { return flag[i] == 0; }
+};
+
+// Reciprocal-space change of every surviving insertion (one lane each).
+template <int NS>
+__global__ void __launch_bounds__(kWBlock) k_widom_recip(WidomArgs w, BoxDev b, EwaldDev ew,
+                                                         const double4 *__restrict__ wk,
+                                                         const double *__restrict__ pos,
+                                                         const int *__restrict__ live, const int *__restrict__ nlive,
+                                                         double *__restrict__ recip) {
+  const int m = *nlive;
+  const int64_t t = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  if ((int64_t)blockIdx.x * kWBlock >= m) return;
+  const bool active = t < m;
+  const int64_t i = live[active ? t : m - 1];
   double px[NS], py[NS], pz[NS];
 #pragma unroll
   for (int a = 0; a < NS; ++a) {
-    const double4 t = sp[i * NS + a];
-    ev += t.x;
-    eq += t.y;
-    ecb += t.z;
-    over = over || t.w != 0.0;
     px[a] = pos[(i * NS + a) * 3];
     py[a] = pos[(i * NS + a) * 3 + 1];
     pz[a] = pos[(i * NS + a) * 3 + 2];
   }
-  double de_recip = 0.0;
-  if (w.charged && lanes_any(active && !over)) {
-    const double d = recip_delta_n<NS>(ew, wk, b, w, px, py, pz);
-    if (!over) de_recip = d;
+  const double d = recip_delta_n<NS>(ew, wk, b, w, px, py, pz);
+  if (active) recip[i] = d;
+}
+
+// widom_var of every insertion: overlapped ones score 0; the others sum
+// their sites' partials in site order and add the reciprocal change.
+template <int NS>
+__global__ void __launch_bounds__(kWBlock) k_widom_finish(WidomArgs w, const int *__restrict__ flag,
+                                                          const double4 *__restrict__ sp,
+                                                          const double *__restrict__ recip, double *part,
+                                                          int64_t *part_ov, double *wvar, double *eout) {
+  const int64_t i0 = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  const bool active = i0 < w.n_ins;
+  const int64_t i = active ? i0 : w.n_ins - 1;
+  const bool over = flag[i] != 0;
+  double ev = 0.0, eq = 0.0, ecb = 0.0, de_recip = 0.0;
+  if (!over) {
+#pragma unroll
+    for (int a = 0; a < NS; ++a) {
+      const double4 t = sp[i * NS + a];
+      ev += t.x;
+      eq += t.y;
+      ecb += t.z;
+    }
+    if (w.charged) de_recip = recip[i];
   }
   widom_finish(w, active, i, ev + eq, ecb, de_recip, over, part, part_ov, wvar, eout);
 }
@@ -807,28 +909,31 @@ static void widom_grid(const cass_box_state &b, int64_t n, int g[3]) {
   for (int d = 0; d < 3; ++d) g[d] = (int)std::min(64.0, std::max(1.0, std::floor(L[d] / side + 0.5)));
 }
 
-typedef void (*place_kernel_t)(WidomArgs, BoxDev, int, int, int, double *, unsigned *, int *);
-typedef void (*combine_kernel_t)(WidomArgs, BoxDev, EwaldDev, const double4 *, const double *, const double4 *,
-                                 double *, int64_t *, double *, double *);
+typedef void (*place_kernel_t)(WidomArgs, BoxDev, int, int, int, double *, unsigned *, int *, int *);
+typedef void (*recip_kernel_t)(WidomArgs, BoxDev, EwaldDev, const double4 *, const double *, const int *,
+                               const int *, double *);
+typedef void (*finish_kernel_t)(WidomArgs, const int *, const double4 *, const double *, double *, int64_t *,
+                                double *, double *);
 typedef void (*real_kernel_t)(RealArgs, const FrameMol *, const double4 *, const int *, const double *, const int *,
-                              const double4 *, const double *, const double *, double4 *);
+                              const double4 *, const double *, const double *, const int *, double4 *);
 
 template <int NS>
-static void split_kernels(place_kernel_t &pl, combine_kernel_t &co) {
+static void split_kernels(place_kernel_t &pl, recip_kernel_t &rc, finish_kernel_t &fi) {
   pl = k_widom_place<NS>;
-  co = k_widom_combine<NS>;
+  rc = k_widom_recip<NS>;
+  fi = k_widom_finish<NS>;
 }
 
-static bool split_kernels_for(int ns, place_kernel_t &pl, combine_kernel_t &co) {
+static bool split_kernels_for(int ns, place_kernel_t &pl, recip_kernel_t &rc, finish_kernel_t &fi) {
   switch (ns) {
-    case 1: split_kernels<1>(pl, co); return true;
-    case 2: split_kernels<2>(pl, co); return true;
-    case 3: split_kernels<3>(pl, co); return true;
-    case 4: split_kernels<4>(pl, co); return true;
-    case 5: split_kernels<5>(pl, co); return true;
-    case 6: split_kernels<6>(pl, co); return true;
-    case 7: split_kernels<7>(pl, co); return true;
-    case 8: split_kernels<8>(pl, co); return true;
+    case 1: split_kernels<1>(pl, rc, fi); return true;
+    case 2: split_kernels<2>(pl, rc, fi); return true;
+    case 3: split_kernels<3>(pl, rc, fi); return true;
+    case 4: split_kernels<4>(pl, rc, fi); return true;
+    case 5: split_kernels<5>(pl, rc, fi); return true;
+    case 6: split_kernels<6>(pl, rc, fi); return true;
+    case 7: split_kernels<7>(pl, rc, fi); return true;
+    case 8: split_kernels<8>(pl, rc, fi); return true;
     default: return false;
   }
 }
@@ -939,6 +1044,14 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   size_t sort_tmp = 0;
   HIP_TRY(e, rocprim::radix_sort_pairs(nullptr, sort_tmp, (unsigned *)nullptr, (unsigned *)nullptr, (int *)nullptr,
                                        (int *)nullptr, (unsigned)nslot, 0, bits, e->stream));
+  if (common) {
+    size_t t1 = 0, t2 = 0;
+    HIP_TRY(e, rocprim::select(nullptr, t1, (int *)nullptr, (int *)nullptr, (int *)nullptr, nslot,
+                               LiveSite{nullptr, ns}, e->stream));
+    HIP_TRY(e, rocprim::select(nullptr, t2, rocprim::counting_iterator<int>(0), (int *)nullptr, (int *)nullptr,
+                               (size_t)n_ins, LiveInsertion{nullptr}, e->stream));
+    sort_tmp = std::max(sort_tmp, std::max(t1, t2));
+  }
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     const size_t o = off;
@@ -962,6 +1075,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const size_t o_pos = carve(common && !place ? sizeof(double) * 3 * n * ns : 0);
   const size_t o_sp = carve(common ? sizeof(double4) * n * ns : 0);
   const size_t o_wk = carve(sizeof(double4) * std::max(b.nvecs, 1));
+  const size_t o_flag = carve(common ? sizeof(int) * n : 0);
+  const size_t o_recip = carve(common ? sizeof(double) * n : 0);
+  const size_t o_cnt = carve(sizeof(int) * 4);
   const size_t o_tmp = carve(sort_tmp);
   char *s = (char *)cass_scratch(e, off);
   if (!s) return cass_fail(e, CASS_ERR_NOMEM, "scratch");
@@ -981,6 +1097,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   double *d_pos = (double *)(s + o_pos);
   double4 *d_sp = (double4 *)(s + o_sp);
   double4 *d_wk = (double4 *)(s + o_wk);
+  int *d_flag = (int *)(s + o_flag);
+  double *d_recip = (double *)(s + o_recip);
+  int *d_cnt = (int *)(s + o_cnt);
   if (!fm.empty()) {
     HIP_TRY(e, hipMemcpyAsync(d_fm, fm.data(), sizeof(FrameMol) * fm.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_fs, fs.data(), sizeof(double4) * fs.size(), hipMemcpyHostToDevice, e->stream));
@@ -1019,19 +1138,19 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   if (common) {
     place_kernel_t pl;
-    combine_kernel_t co;
-    if (!split_kernels_for(ns, pl, co)) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+    recip_kernel_t rk;
+    finish_kernel_t fk;
+    if (!split_kernels_for(ns, pl, rk, fk)) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
     const double *pos = place ? d_place : d_pos;
     hipLaunchKernelGGL(pl, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, w, bv, g[0], g[1], g[2],
-                       d_pos, k0, v0);
+                       d_pos, k0, v0, d_flag);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, rocprim::radix_sort_pairs((void *)(s + o_tmp), sort_tmp, k0, k1, v0, v1, (unsigned)nslot, 0, bits,
                                          e->stream));
-    HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
-    const unsigned rblk = (unsigned)((nslot + kWBlock - 1) / kWBlock);
     RealArgs ra;
     std::memset(&ra, 0, sizeof(ra));
-    ra.nslot = (int64_t)nslot;
+    ra.n_ins = (int64_t)n;
+    ra.npad = (int64_t)((n + 63) / 64 * 64);
     ra.ns = ns;
     ra.ntypes = nt;
     ra.nmols = (int)fm.size();
@@ -1053,13 +1172,30 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
       ra.hL[d] = pp.hlength[d];
       ra.Li[d] = 1.0 / pp.basis_length[d];
     }
-    const size_t shbytes = sizeof(double4) * tab2.size() + sizeof(double) * etab.size();
+    // hard overlaps, then the surviving sites (spatial order kept) and
+    // insertions (index order)
+    const unsigned rblk = (unsigned)((ns * ((n + 63) / 64 * 64) + kWBlock - 1) / kWBlock);
+    hipLaunchKernelGGL(k_widom_overlap, dim3(rblk), dim3(kWBlock), 0, e->stream, ra, d_fm, d_fs, pos, v1, d_flag);
+    HIP_TRY(e, hipGetLastError());
+    int *live_sites = v0;
+    int *live_ins = (int *)k1;
+    HIP_TRY(e, rocprim::select((void *)(s + o_tmp), sort_tmp, v1, live_sites, d_cnt, nslot, LiveSite{d_flag, ns},
+                               e->stream));
+    HIP_TRY(e, rocprim::select((void *)(s + o_tmp), sort_tmp, rocprim::counting_iterator<int>(0), live_ins,
+                               d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
+    HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
+    const size_t shbytes = sizeof(double) * std::max<size_t>(etab.size(), 1);
     hipLaunchKernelGGL(real_kernel(e->prec), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_ft,
-                       pos, v1, d_tab2, d_q, d_etab, d_sp);
+                       pos, live_sites, d_tab2, d_q, d_etab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
-    hipLaunchKernelGGL(co, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, ew, d_wk, pos, d_sp, part, part_ov,
-                       d_var, d_eout);
+    if (w.charged && ew.nvecs > 0) {
+      hipLaunchKernelGGL(rk, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, ew, d_wk, pos, live_ins, d_cnt + 1,
+                         d_recip);
+      HIP_TRY(e, hipGetLastError());
+    }
+    hipLaunchKernelGGL(fk, dim3(nblk), dim3(kWBlock), 0, e->stream, w, d_flag, d_sp, d_recip, part, part_ov, d_var,
+                       d_eout);
     HIP_TRY(e, hipGetLastError());
   } else {
     WidomPairTab pt{d_tab, nt};
