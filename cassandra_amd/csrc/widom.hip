@@ -384,88 +384,87 @@ __device__ inline double exp_tab(double z, const double *t, int n) {
   return t[k] * p;
 }
 
-// One pair of the real-space site kernel: LJ (cut, cut_tail, cut_shift: the
-// table holds 4 eps, sigma^2 and the shift 4 eps (c^12 - c^6)) and the
-// Ewald or plain Coulomb term, in the precision mode's arithmetic.
+// The pair terms of the real-space site kernel in the precision mode's
+// arithmetic.  LJ (cut, cut_tail, cut_shift): the row holds 4 eps, sigma^2
+// and the shift 4 eps (c^12 - c^6).
 template <typename P>
-__device__ inline void site_pair_energy(const RealArgs &ra, double r2, double4 prm, double qq, const double *etab,
-                                        bool do_v, bool do_q, double &e_v, double &e_q) {
+__device__ inline void site_pair_lj(double r2, double4 prm, double &e_v) {
+  if constexpr (std::is_same<P, PrecFP64>::value) {
+    // the reference's operations (lj_cut with MathRef)
+    const double s2 = prm.y / r2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    e_v = prm.x * (s12 - s6) - prm.z;
+  } else if constexpr (std::is_same<P, PrecMixed>::value) {
+    // v_rcp_f64 refined by one Newton step
+    double y = __builtin_amdgcn_rcp(r2);
+    y = fma(y, fma(-r2, y, 1.0), y);
+    const double s2 = prm.y * y, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    e_v = prm.x * (s12 - s6) - prm.z;
+  } else {
+    const float s2 = (float)prm.y * __builtin_amdgcn_rcpf((float)r2), s6 = s2 * s2 * s2, s12 = s6 * s6;
+    e_v = (double)((float)prm.x * (s12 - s6) - (float)prm.z);
+  }
+}
+
+// Ewald real space (A&S erfc) or plain Coulomb.
+template <typename P, int QS>
+__device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, const double *etab, double &e_q) {
   const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027, A5 = 1.061405429,
                PA = 0.3275911;
   if constexpr (std::is_same<P, PrecFP64>::value) {
-    // the reference's operations (lj_cut / ewald_real with MathRef)
-    if (do_v) {
-      const double s2 = prm.y / r2, s6 = s2 * s2 * s2, s12 = s6 * s6;
-      e_v = prm.x * (s12 - s6) - prm.z;
-    }
-    if (do_q) {
-      const double rij = sqrt(r2);
-      if (ra.q_style == CASS_Q_EWALD) {
-        const double ec = erfc_as<double, MathRef>(ra.alpha * rij);
-        const double eqq = qq / rij * kChargeFactor;
-        e_q = eqq - (1.0 - ec) * eqq;
-      } else {
-        e_q = kChargeFactor * qq / rij;
-      }
+    // the reference's operations (ewald_real / coul_cut with MathRef)
+    const double rij = sqrt(r2);
+    if (QS == CASS_Q_EWALD) {
+      const double ec = erfc_as<double, MathRef>(ra.alpha * rij);
+      const double eqq = qq / rij * kChargeFactor;
+      e_q = eqq - (1.0 - ec) * eqq;
+    } else {
+      e_q = kChargeFactor * qq / rij;
     }
   } else if constexpr (std::is_same<P, PrecMixed>::value) {
-    // FP64 on hardware seeds: v_rcp_f64 / v_rsq_f64 refined by one Newton
-    // step, exp from the table
-    if (do_v) {
-      double y = __builtin_amdgcn_rcp(r2);
-      y = fma(y, fma(-r2, y, 1.0), y);
-      const double s2 = prm.y * y, s6 = s2 * s2 * s2, s12 = s6 * s6;
-      e_v = prm.x * (s12 - s6) - prm.z;
-    }
-    if (do_q) {
-      double ri = __builtin_amdgcn_rsq(r2);
-      ri = fma(0.5 * ri, fma(-r2 * ri, ri, 1.0), ri);
-      if (ra.q_style == CASS_Q_EWALD) {
-        const double x = ra.alpha * (r2 * ri);
-        const double den = fma(PA, x, 1.0);
-        double t = __builtin_amdgcn_rcp(den);
-        t = fma(t, fma(-den, t, 1.0), t);
-        const double tp = t * fma(t, fma(t, fma(t, fma(t, A5, A4), A3), A2), A1);
-        e_q = qq * kChargeFactor * ri * (tp * exp_tab(-(x * x), etab, ra.nexp));
-      } else {
-        e_q = kChargeFactor * qq * ri;
-      }
+    // v_rsq_f64 / v_rcp_f64 refined by one Newton step, exp from the table
+    double ri = __builtin_amdgcn_rsq(r2);
+    ri = fma(0.5 * ri, fma(-r2 * ri, ri, 1.0), ri);
+    if (QS == CASS_Q_EWALD) {
+      const double x = ra.alpha * (r2 * ri);
+      const double den = fma(PA, x, 1.0);
+      double t = __builtin_amdgcn_rcp(den);
+      t = fma(t, fma(-den, t, 1.0), t);
+      const double tp = t * fma(t, fma(t, fma(t, fma(t, A5, A4), A3), A2), A1);
+      e_q = qq * kChargeFactor * ri * (tp * exp_tab(-(x * x), etab, ra.nexp));
+    } else {
+      e_q = kChargeFactor * qq * ri;
     }
   } else {
-    // single precision on the hardware instructions
     const float r2f = (float)r2;
-    if (do_v) {
-      const float s2 = (float)prm.y * __builtin_amdgcn_rcpf(r2f), s6 = s2 * s2 * s2, s12 = s6 * s6;
-      e_v = (double)((float)prm.x * (s12 - s6) - (float)prm.z);
-    }
-    if (do_q) {
-      const float ri = __builtin_amdgcn_rsqf(r2f);
-      if (ra.q_style == CASS_Q_EWALD) {
-        const float x = (float)ra.alpha * (r2f * ri);
-        const float t = __builtin_amdgcn_rcpf(fmaf((float)PA, x, 1.0f));
-        const float tp =
-            t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, (float)A5, (float)A4), (float)A3), (float)A2), (float)A1);
-        e_q = (double)((float)(qq * kChargeFactor) * ri * (tp * __expf(-(x * x))));
-      } else {
-        e_q = (double)((float)(kChargeFactor * qq) * ri);
-      }
+    const float ri = __builtin_amdgcn_rsqf(r2f);
+    if (QS == CASS_Q_EWALD) {
+      const float x = (float)ra.alpha * (r2f * ri);
+      const float t = __builtin_amdgcn_rcpf(fmaf((float)PA, x, 1.0f));
+      const float tp = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, (float)A5, (float)A4), (float)A3), (float)A2), (float)A1);
+      e_q = (double)((float)(qq * kChargeFactor) * ri * (tp * __expf(-(x * x))));
+    } else {
+      e_q = (double)((float)(kChargeFactor * qq) * ri);
     }
   }
 }
 
 // Pair terms of one site (lane) against frame molecule m.  The wave holds
-// sites of one kind (a, qa, the LJ row are uniform), so the choice of terms
-// per frame site is a scalar branch.  UNI: every site of m takes the image
-// shift of m's centre seen from the wave centre c, S = -L rint((c - com)/L).
-// When (wave extent + cutoff + max_dcom) is below half the box, S is the
-// per-pair minimum image of minimum_image_separation.f90:35-60 for every
-// pair inside the cutoff, and any other pair stays outside the cutoff;
-// (p - s) + S is then the reference's own separation, bit for bit.
-template <typename P, bool UNI>
+// sites of one kind (qa and the LJ row are uniform), so the choice of terms
+// per frame site is a scalar branch.  The sites reaching this kernel have no
+// hard overlap (k_widom_overlap removed those insertions), so every lane
+// computes the pair and the cutoff tests select what is added: no per-lane
+// branches.  QS: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT; SAME: the vdW and
+// Coulomb cutoffs both equal rc_pair.  UNI: every site of m takes the image
+// shift of m's centre seen from the wave centre c, S = -L rint((c - com)/L);
+// when (wave extent + cutoff + max_dcom) is below half the box, S is the
+// per-pair minimum image of minimum_image_separation.f90:35-60 for every pair
+// inside the cutoff, and any other pair stays outside the cutoff, so
+// (p - s) + S is the reference's own separation, bit for bit.
+template <typename P, int QS, bool SAME, bool UNI>
 __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
                                   const int *__restrict__ ftype, const double4 *__restrict__ lj_row,
                                   const double *etab, double cx, double cy, double cz, bool live, double qa,
-                                  double px, double py, double pz, double &ev, double &eq, double &ecb, bool &over) {
+                                  double px, double py, double pz, double &ev, double &eq, double &ecb) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (UNI) {
     sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
@@ -481,9 +480,9 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
       dy += sy;
       dz += sz;
     } else {
-      if (dx > ra.hL[0]) dx -= ra.L[0]; else if (dx < -ra.hL[0]) dx += ra.L[0];
-      if (dy > ra.hL[1]) dy -= ra.L[1]; else if (dy < -ra.hL[1]) dy += ra.L[1];
-      if (dz > ra.hL[2]) dz -= ra.L[2]; else if (dz < -ra.hL[2]) dz += ra.L[2];
+      dx = fma(-rint(dx * ra.Li[0]), ra.L[0], dx);
+      dy = fma(-rint(dy * ra.Li[1]), ra.L[1], dy);
+      dz = fma(-rint(dz * ra.Li[2]), ra.L[2], dz);
     }
     const double r2 = dx * dx + dy * dy + dz * dz;
     const bool near = live && r2 <= ra.rc_pair2;
@@ -492,29 +491,26 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
     const double4 prm = tj >= 0 ? lj_row[tj] : make_double4(0.0, 0.0, 0.0, 0.0);
     const double qq = qa * st.w;
     const bool has_v = prm.x != 0.0;  // eps = 0 (or no type) adds exactly 0
-    const bool has_q = ra.q_style != CASS_Q_NONE && qq != 0.0;
-    if (near) {
-      over = over || r2 < ra.rc_low2;
-      const bool incb = r2 <= ra.rc_cbmc2;
-      const bool gv = r2 <= ra.rc_vdw2, gq = r2 <= ra.rc_coul2;
-      double e_v = 0.0, e_q = 0.0;
-      site_pair_energy<P>(ra, r2, prm, qq, etab, has_v, has_q, e_v, e_q);
-      if (ra.same_cut) {
-        ev += e_v;
-        eq += e_q;
-      } else {
-        if (gv) ev += e_v;
-        if (gq) eq += e_q;
-      }
-      if (incb) ecb += e_v + (ra.coul_cbmc ? e_q : 0.0);
+    const bool has_q = QS != CASS_Q_NONE && qq != 0.0;
+    double e_v = 0.0, e_q = 0.0;
+    if (has_v) site_pair_lj<P>(r2, prm, e_v);
+    if (has_q) site_pair_qq<P, QS>(ra, r2, qq, etab, e_q);
+    const bool incb = near && r2 <= ra.rc_cbmc2;
+    if (SAME) {
+      ev += near ? e_v : 0.0;
+      eq += near ? e_q : 0.0;
+    } else {
+      ev += (near && r2 <= ra.rc_vdw2) ? e_v : 0.0;
+      eq += (near && r2 <= ra.rc_coul2) ? e_q : 0.0;
     }
+    ecb += incb ? e_v + (ra.coul_cbmc ? e_q : 0.0) : 0.0;
   }
 }
 
 // One lane per site of the surviving insertions.  Slots are laid out kind by
 // kind, npad (a multiple of 64) per kind, so every wavefront holds sites of a
 // single kind.
-template <typename P>
+template <typename P, int QS, bool SAME>
 __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const FrameMol *__restrict__ fmol,
                                                         const double4 *__restrict__ fsite,
                                                         const int *__restrict__ ftype, const double *__restrict__ pos,
@@ -540,8 +536,6 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   const double4 *lj_row = ptab + (int64_t)a * ra.ntypes;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
   double ev = 0.0, eq = 0.0, ecb = 0.0;
-  bool over = false;
-  bool dead = !active;
   // wave centre (first lane's site) and the wave's extent around it
   const double cx = wave_uniform(px), cy = wave_uniform(py), cz = wave_uniform(pz);
   double dx = px - cx, dy = py - cy, dz = pz - cz;
@@ -551,7 +545,7 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   dz -= ra.L[2] * rint(dz * ra.Li[2]);
   const double rad = sqrt(wave_max(dx * dx + dy * dy + dz * dz));
   const bool uni = ext + ra.rc_pair + ra.dcom_max + 1e-3 < fmin(ra.hL[0], fmin(ra.hL[1], ra.hL[2]));
-  for (int base = 0; base < ra.nmols && wave_any(!dead); base += 64) {
+  for (int base = 0; base < ra.nmols; base += 64) {
     const int jt = base + (int)(threadIdx.x & 63);
     bool cand = false;
     if (jt < ra.nmols) {
@@ -567,17 +561,16 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
     while (mask) {
       const int j = base + __builtin_ctzll(mask);
       mask &= mask - 1;
-      const FrameMol m = fmol[j];
+      const FrameMol fm = fmol[j];
       if (uni)
-        site_pairs<P, true>(ra, m, fsite, ftype, lj_row, etab, cx, cy, cz, !dead, qa, px, py, pz, ev, eq, ecb, over);
+        site_pairs<P, QS, SAME, true>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, qa, px, py, pz, ev,
+                                      eq, ecb);
       else
-        site_pairs<P, false>(ra, m, fsite, ftype, lj_row, etab, cx, cy, cz, !dead, qa, px, py, pz, ev, eq, ecb,
-                             over);
-      dead = dead || over;
-      if (!wave_any(!dead)) break;
+        site_pairs<P, QS, SAME, false>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, qa, px, py, pz, ev,
+                                       eq, ecb);
     }
   }
-  if (active) out[v] = make_double4(ev, eq, ecb, over ? 1.0 : 0.0);
+  if (active) out[v] = make_double4(ev, eq, ecb, 0.0);
 }
 
 // widom_var and the block sums of one insertion per lane (tail of the
@@ -938,10 +931,22 @@ static bool split_kernels_for(int ns, place_kernel_t &pl, recip_kernel_t &rc, fi
   }
 }
 
-static real_kernel_t real_kernel(int prec) {
-  if (prec == CASS_PREC_MIXED) return k_widom_real<PrecMixed>;
-  if (prec == CASS_PREC_FP32) return k_widom_real<PrecFP32>;
-  return k_widom_real<PrecFP64>;
+template <typename P, int QS>
+static real_kernel_t real_kernel_q(bool same) {
+  return same ? k_widom_real<P, QS, true> : k_widom_real<P, QS, false>;
+}
+
+template <typename P>
+static real_kernel_t real_kernel_p(int q_style, bool same) {
+  if (q_style == CASS_Q_EWALD) return real_kernel_q<P, CASS_Q_EWALD>(same);
+  if (q_style == CASS_Q_CUT) return real_kernel_q<P, CASS_Q_CUT>(same);
+  return real_kernel_q<P, CASS_Q_NONE>(same);
+}
+
+static real_kernel_t real_kernel(int prec, int q_style, bool same) {
+  if (prec == CASS_PREC_MIXED) return real_kernel_p<PrecMixed>(q_style, same);
+  if (prec == CASS_PREC_FP32) return real_kernel_p<PrecFP32>(q_style, same);
+  return real_kernel_p<PrecFP64>(q_style, same);
 }
 
 int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed, const double *place,
@@ -1185,7 +1190,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
                                d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
     HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
     const size_t shbytes = sizeof(double) * std::max<size_t>(etab.size(), 1);
-    hipLaunchKernelGGL(real_kernel(e->prec), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_ft,
+    hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_ft,
                        pos, live_sites, d_tab2, d_q, d_etab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
