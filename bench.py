@@ -38,8 +38,8 @@ def widom_frame():
 
 
 def pair_counts(system, sp_index, n_sample, seed):
-    """Per attempted insertion (uniform random placements): the LJ and q-q
-    site pairs inside the cutoff that the algorithm must evaluate, and the
+    """Per attempted insertion (uniform random placements): site pairs inside
+    the cutoff by kind -- LJ and Coulomb, Coulomb only, LJ only -- and the
     fraction of insertions that survive the hard-overlap test.  An insertion
     with a site pair closer than rcut_low is rejected on that pair, as the
     reference's Build_Molecule does, so its pairs are not counted."""
@@ -55,7 +55,7 @@ def pair_counts(system, sp_index, n_sample, seed):
     ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
     rc2 = system.rcut_vdw ** 2
     rlow2 = system.rcut_low ** 2
-    n_lj = n_q = 0
+    counts = np.zeros(3)
     n_live = 0
     for _ in range(n_sample):
         c = (rng.random(3) - 0.5) * L
@@ -70,37 +70,34 @@ def pair_counts(system, sp_index, n_sample, seed):
         n_live += 1
         for a, at in enumerate(probe.atoms):
             inr = r2s[a] <= rc2
-            if at.eps_K > 0:
-                n_lj += int((inr & lj).sum())
-            if at.charge != 0:
-                n_q += int((inr & (q != 0)).sum())
-    return n_lj / n_sample, n_q / n_sample, n_live / n_sample
+            v = inr & lj & (at.eps_K > 0)
+            e = inr & (q != 0) & (at.charge != 0)
+            counts += [(v & e).sum(), (e & ~v).sum(), (v & ~e).sum()]
+    return counts / n_sample, n_live / n_sample
 
 
-# Algorithmic flops per unit (an FMA counts 2; sqrt, division and the exp of
-# the A&S erfc count 1, 1 and 4):
-#   pair geometry (difference, minimum image, r^2)            11 per pair
-#   LJ 12-6 (s^2, s^6, s^12, 4 eps (s^12 - s^6))                8 per LJ pair
-#   Ewald real space (r, t = 1/(1 + p a r), A&S polynomial,
-#                     exp(-a^2 r^2), q_i q_j / r)              23 per q-q pair
-#   reciprocal change: phase step (complex multiply) and the
-#                     dC, dS sums                              10 per charged probe site per k
-#                     Cn (2 C dC + dC^2 + 2 S dS + dS^2)       10 per k
-FL_GEOM, FL_LJ, FL_EWALD, FL_KSITE, FL_K = 11.0, 8.0, 23.0, 10.0, 10.0
+# Algorithmic flops per unit, in the weights of the hot-path scope table
+# (add/mul/cmp/select 1, FMA 2, rcp/sqrt 4, FP32 exp 4, FP64 exp 20):
+#   site pair inside the cutoff, minimum image and r^2 included:
+#     LJ + Ewald real space 94 (FP64) / 78 (FP32); Coulomb only 82 / 66;
+#     LJ only 26 / 26
+#   reciprocal change by phase recurrence (no sincos per k): 8 per charged
+#     probe site per k (complex step 6, dC/dS sums 2) + 9 per k
+#     (Cn (dC^2 + dS^2) + 2 Cn C dC + 2 Cn S dS)
+FL_PAIR64 = (94.0, 82.0, 26.0)
+FL_PAIR32 = (78.0, 66.0, 26.0)
+FL_KSITE, FL_K = 8.0, 9.0
 
 
-def algorithmic_flops(prec, n_lj, n_q, n_probe_sites, K, f_live=1.0):
+def algorithmic_flops(prec, pairs, n_probe_sites, K, f_live=1.0):
     """Per attempted insertion: {"real": (FP32, FP64), "recip": (FP32, FP64)}
     flops of the real-space pair kernel and of the reciprocal kernel (only
     the fraction f_live of insertions that survive the overlap test needs
-    the reciprocal change).  Pair
-    geometry and the reciprocal change are FP64 in every mode; the pair terms
-    are FP32 in the fp32 mode only (mixed refines FP32 hardware seeds in
-    FP64)."""
+    the reciprocal change).  The pair terms are FP32 in the fp32 mode only."""
     from cassandra_amd import engine as E
-    pair = FL_LJ * n_lj + FL_EWALD * n_q
-    geom = FL_GEOM * max(n_lj, n_q)
-    real = (pair, geom) if prec == E.FP32 else (0.0, pair + geom)
+    w = FL_PAIR32 if prec == E.FP32 else FL_PAIR64
+    pair = float(np.dot(w, pairs))
+    real = (pair, 0.0) if prec == E.FP32 else (0.0, pair)
     recip = (0.0, f_live * (FL_KSITE * n_probe_sites + FL_K) * K)
     return {"real": real, "recip": recip}
 
@@ -270,7 +267,7 @@ def main():
                       "overlap_flags_equal": bool(np.array_equal(w32 > 0, w64 > 0))}
     log(f"dE errors {errs}")
 
-    n_lj, n_q, f_live = pair_counts(system, sp, 400, 5)
+    pairs, f_live = pair_counts(system, sp, 400, 5)
     n_charged = sum(1 for at in probe.atoms if at.charge != 0)
 
     def roof1(f32, f64, ms, label):
@@ -282,15 +279,28 @@ def main():
                 "traffic": None, "kernel": label, "kernel_ms": ms,
                 "flops_per_insertion": {"fp32": f32, "fp64": f64}}
 
+    traffic = {}
+    tpath = os.path.join(ROOT, "profiles", "widom_co2_spce512_mixed_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f)
+
     def roof(prec, stages, tag):
         """Roofline of the dominant kernel of the batch (by device time), the
         other kernel and the whole batch."""
-        fl = algorithmic_flops(prec, n_lj, n_q, n_charged, K, f_live)
+        fl = algorithmic_flops(prec, pairs, n_charged, K, f_live)
         r_real = roof1(*fl["real"], stages[1], f"k_widom_real<{tag}>")
         r_recip = roof1(*fl["recip"], stages[2], "k_widom_recip<3> + k_widom_finish<3>")
         tot32 = fl["real"][0] + fl["recip"][0]
         tot64 = fl["real"][1] + fl["recip"][1]
         r_all = roof1(tot32, tot64, stages[3], "whole batch (place, sort, overlap, select, real, recip, finish)")
+        if prec == E.MIXED and traffic and n == N_INS:
+            # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs of
+            # this workload, scaled by the copy-kernel calibration
+            for r, key in ((r_real, "k_widom_real"), (r_recip, "k_widom_recip")):
+                r["traffic"] = traffic["kernels"][key]["bytes"]
+                r["traffic_unit"] = "bytes per launch"
+                r["traffic_source"] = traffic["source"]
         dom, other = (r_real, r_recip) if stages[1] >= stages[2] else (r_recip, r_real)
         dom = dict(dom)
         dom["other_kernel"] = other
@@ -300,7 +310,8 @@ def main():
         return dom
 
     roofline = roof(E.MIXED, k_mixed, "PrecMixed")
-    roofline["pairs_in_cutoff_per_insertion"] = {"lj": n_lj, "qq": n_q}
+    roofline["pairs_in_cutoff_per_insertion"] = {"lj_and_coulomb": pairs[0], "coulomb_only": pairs[1],
+                                                 "lj_only": pairs[2]}
     roofline["surviving_fraction"] = f_live
     roofline["kvectors"] = K
 
