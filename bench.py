@@ -191,6 +191,69 @@ def nvt_moves_per_s(nsteps=2000):
     return nsteps / dt
 
 
+def widom_configs(n, steps):
+    """The other probe/frame pairs of the Widom configuration (mixed mode):
+    trial energies/s and stage times per pair."""
+    from cassandra_amd import engine as E
+    from cassandra_amd import systems as S
+    out = {}
+    for fname, frame_fn in (("spce512", S.config_spce512), ("lj2000", S.config_lj2000)):
+        for pname, probe_fn in (("ch4", S.species_methane), ("co2", S.species_co2)):
+            system = S.with_probe(frame_fn(), probe_fn()).parsed()
+            sp = len(system.species) - 1
+            eng = E.Engine(system, precision=E.MIXED)
+            box = eng.add_box(system, max_mols=sum(system.nmols) + 1)
+            if system.charge_style == "ewald":
+                box.ewald_setup()
+            box.compute_system_total_energy()
+            box.widom_set_template(sp, system.species[sp].ref_coords)
+            dt, st = time_widom(box, sp, n, 99, steps, 1)
+            r, _ = box.widom_batch(sp, n, 7)
+            out[f"{pname}_in_{fname}"] = {"trial_energies_per_s": n * steps / dt, "stage_ms": st,
+                                         "overlap_fraction": r.n_overlaps / n}
+            eng.close()
+    return out
+
+
+def cbmc_gcmc(nboxes=48, steps=10):
+    """Lock-step CBMC GCMC of n-pentane in 48 boxes (one insertion or
+    deletion attempt per box per step; kappa_ins 12, kappa_dih 24; trial
+    sets of all boxes in one hipGraph-replayed launch per CBMC stage)."""
+    from cassandra_amd import cbmc
+    from cassandra_amd import engine as E
+    d = cbmc.MultiBoxCBMC(nboxes=nboxes, seed=2024, precision=E.MIXED, graph=True, autotune=True)
+    for _ in range(2):
+        d.step()
+    t0 = d.stats["trial_energies"]
+    t = time.perf_counter()
+    for _ in range(steps):
+        d.step()
+    dt = time.perf_counter() - t
+    trials = d.stats["trial_energies"] - t0
+    mols = int(sum(int(b.live.sum()) for b in d.boxes))
+    d.close()
+    return {"moves_per_s": nboxes * steps / dt, "trial_energies_per_s": trials / dt, "boxes": nboxes,
+            "molecules": mols, "autotuned_block": d.tile}
+
+
+def adjoint_frames(frames=20):
+    """cass_adjoint on the CO2/N2 mixture (400 + 400 molecules, 40 A box,
+    Ewald): frames/s of energy + dE/d(eps, sigma, q) + forces."""
+    from cassandra_amd import engine as E
+    from cassandra_amd import systems as S
+    system = S.config_co2n2().parsed()
+    eng = E.Engine(system, precision=E.FP64)
+    box = eng.add_box(system)
+    box.ewald_setup()
+    box.adjoint()
+    t = time.perf_counter()
+    for _ in range(frames):
+        box.adjoint()
+    dt = time.perf_counter() - t
+    eng.close()
+    return {"frames_per_s": frames / dt, "sites": int(system.nsites())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +384,12 @@ def main():
         log(f"plain fp64 per launch: {extra['plain_fp64_per_launch_trials_per_s']:.4g} trials/s")
         extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(1000)
         log(f"nvt: {extra['nvt_spce512_moves_per_s']:.4g} moves/s")
+        extra["widom_other_pairs"] = widom_configs(n, 3)
+        log(f"widom pairs: { {k: round(v['trial_energies_per_s'] / 1e6, 1) for k, v in extra['widom_other_pairs'].items()} } M/s")
+        extra["cbmc_gcmc_48_boxes"] = cbmc_gcmc()
+        log(f"cbmc gcmc: {extra['cbmc_gcmc_48_boxes']}")
+        extra["adjoint_co2_n2"] = adjoint_frames()
+        log(f"adjoint: {extra['adjoint_co2_n2']}")
     cpu = None
     if not a.no_cpu:
         rate, ns, dt = cpu_baseline(system, sp, seconds=8.0)
