@@ -368,20 +368,27 @@ __device__ inline double wave_uniform(double v) {
 
 __device__ inline bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
 
-// exp(z), z <= 0: exp(-k/32) from the table times a degree-5 Taylor factor
-// of the remainder |d| <= 1/64 (relative error < 3e-14).  Lanes past the
-// table take the library exp.
+// a*b + c as one v_fma_f64 with all operands in VGPRs: keeps constant
+// coefficients of the polynomial chains below in registers instead of the
+// copy that the two-address v_fmac form needs.
+__device__ inline double fma_v(double a, double b, double c) {
+  double r;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+  return r;
+}
+
+// exp(z) for -(alpha rc_pair)^2 <= z <= 0: exp(-k/64) from the table times a
+// degree-4 Taylor factor of the remainder |d| <= 1/128 (relative error
+// < 3e-13).  Lanes outside the cutoff, whose value is discarded, are clamped
+// to the table.
 __device__ inline double exp_tab(double z, const double *t, int n) {
-  const double kf = rint(-32.0 * z);
-  const double d = fma(kf, 0.03125, z);
-  double p = fma(d, 1.0 / 120.0, 1.0 / 24.0);
-  p = fma(d, p, 1.0 / 6.0);
-  p = fma(d, p, 0.5);
-  p = fma(d, p, 1.0);
-  p = fma(d, p, 1.0);
-  const int k = (int)kf;
-  if (wave_any(k >= n)) return k < n ? t[k] * p : exp(z);
-  return t[k] * p;
+  const double kf = rint(-64.0 * z);
+  const double d = fma(kf, 0.015625, z);
+  double p = fma_v(d, 1.0 / 24.0, 1.0 / 6.0);
+  p = fma_v(d, p, 0.5);
+  p = fma_v(d, p, 1.0);
+  p = fma_v(d, p, 1.0);
+  return t[min((int)kf, n - 1)] * p;
 }
 
 // The pair terms of the real-space site kernel in the precision mode's
@@ -407,7 +414,8 @@ __device__ inline void site_pair_lj(double r2, double4 prm, double &e_v) {
 
 // Ewald real space (A&S erfc) or plain Coulomb.
 template <typename P, int QS>
-__device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, const double *etab, double &e_q) {
+__device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, double qqK, const double *etab,
+                                    double &e_q) {
   const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027, A5 = 1.061405429,
                PA = 0.3275911;
   if constexpr (std::is_same<P, PrecFP64>::value) {
@@ -429,10 +437,11 @@ __device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, co
       const double den = fma(PA, x, 1.0);
       double t = __builtin_amdgcn_rcp(den);
       t = fma(t, fma(-den, t, 1.0), t);
-      const double tp = t * fma(t, fma(t, fma(t, fma(t, A5, A4), A3), A2), A1);
-      e_q = qq * kChargeFactor * ri * (tp * exp_tab(-(x * x), etab, ra.nexp));
+      const double tp = t * fma_v(t, fma_v(t, fma_v(t, fma_v(t, A5, A4), A3), A2), A1);
+      const double ex = ra.nexp > 0 ? exp_tab(-(x * x), etab, ra.nexp) : exp(-(x * x));
+      e_q = qqK * ri * (tp * ex);
     } else {
-      e_q = kChargeFactor * qq * ri;
+      e_q = qqK * ri;
     }
   } else {
     const float r2f = (float)r2;
@@ -463,8 +472,10 @@ __device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, co
 template <typename P, int QS, bool SAME, bool UNI>
 __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
                                   const int *__restrict__ ftype, const double4 *__restrict__ lj_row,
-                                  const double *etab, double cx, double cy, double cz, bool live, double qa,
-                                  double px, double py, double pz, double &ev, double &eq, double &ecb) {
+                                  const double *etab, double cx, double cy, double cz, bool live,
+                                  uint64_t live_mask, double qa, double qaK, bool qa_nz, double px, double py,
+                                  double pz,
+                                  double &ev, double &eq, double &ecb) {
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (UNI) {
     sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
@@ -473,7 +484,8 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
   }
   for (int k = m.first; k < m.first + m.n; ++k) {
     const double4 st = fsite[k];
-    const int tj = ftype[k];
+    const int code = ftype[k];  // (atom type + 1) | charged << 16
+    const int tj = (code & 0xffff) - 1;
     double dx = px - st.x, dy = py - st.y, dz = pz - st.z;
     if (UNI) {
       dx += sx;
@@ -485,16 +497,17 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
       dz = fma(-rint(dz * ra.Li[2]), ra.L[2], dz);
     }
     const double r2 = dx * dx + dy * dy + dz * dz;
-    const bool near = live && r2 <= ra.rc_pair2;
-    if (!wave_any(near)) continue;
-    // uniform per frame site: the LJ row entry and the charge product
+    const bool inr = r2 <= ra.rc_pair2;
+    if ((__builtin_amdgcn_ballot_w64(inr) & live_mask) == 0ull) continue;
+    const bool near = inr && live;
+    // uniform per frame site: the LJ row entry and whether the pair is charged
     const double4 prm = tj >= 0 ? lj_row[tj] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const double qq = qa * st.w;
     const bool has_v = prm.x != 0.0;  // eps = 0 (or no type) adds exactly 0
-    const bool has_q = QS != CASS_Q_NONE && qq != 0.0;
+    const bool has_q = QS != CASS_Q_NONE && qa_nz && (code >> 16) != 0;
+    const double qq = qa * st.w;
     double e_v = 0.0, e_q = 0.0;
     if (has_v) site_pair_lj<P>(r2, prm, e_v);
-    if (has_q) site_pair_qq<P, QS>(ra, r2, qq, etab, e_q);
+    if (has_q) site_pair_qq<P, QS>(ra, r2, qq, qaK * st.w, etab, e_q);
     const bool incb = near && r2 <= ra.rc_cbmc2;
     if (SAME) {
       ev += near ? e_v : 0.0;
@@ -523,7 +536,7 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   const int64_t m = *nlive_sites / ra.ns;
   const int64_t npad = (m + 63) / 64 * 64;
   if ((int64_t)blockIdx.x * kWBlock >= (int64_t)ra.ns * npad) return;
-  extern __shared__ double etab[];  // exp(-k/32)
+  extern __shared__ double etab[];  // exp(-k/64)
   for (int t = threadIdx.x; t < ra.nexp; t += blockDim.x) etab[t] = etab_g[t];
   __syncthreads();
   const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
@@ -533,6 +546,9 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   if (!wave_any(active)) return;
   const int v = order[(int64_t)a * m + (active ? idx : m - 1)];
   const double qa = site_q[a];
+  const bool qa_nz = qa != 0.0;
+  const double qaK = qa * kChargeFactor;
+  const uint64_t live_mask = __builtin_amdgcn_ballot_w64(active);
   const double4 *lj_row = ptab + (int64_t)a * ra.ntypes;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
   double ev = 0.0, eq = 0.0, ecb = 0.0;
@@ -563,11 +579,11 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
       mask &= mask - 1;
       const FrameMol fm = fmol[j];
       if (uni)
-        site_pairs<P, QS, SAME, true>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, qa, px, py, pz, ev,
-                                      eq, ecb);
+        site_pairs<P, QS, SAME, true>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, live_mask, qa, qaK, qa_nz,
+                                      px, py, pz, ev, eq, ecb);
       else
-        site_pairs<P, QS, SAME, false>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, qa, px, py, pz, ev,
-                                       eq, ecb);
+        site_pairs<P, QS, SAME, false>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, live_mask, qa,
+                                       qaK, qa_nz, px, py, pz, ev, eq, ecb);
     }
   }
   if (active) out[v] = make_double4(ev, eq, ecb, 0.0);
@@ -993,7 +1009,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   // frame tables (host mirrors -> device)
   std::vector<FrameMol> fm;
   std::vector<double4> fs;
-  std::vector<int> ft;
+  std::vector<int> ft, fcode;
   fm.reserve(b.nmols);
   for (int m = 0; m < b.nmols; ++m) {
     if (!b.live[m]) continue;
@@ -1008,6 +1024,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     for (int a = 0; a < nat; ++a) {
       fs.push_back(make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], e->atom_charge[off + a]));
       ft.push_back(e->atom_type[off + a]);
+      fcode.push_back((e->atom_type[off + a] + 1) | (e->atom_charge[off + a] != 0.0 ? 0x10000 : 0));
     }
   }
   const int nt = std::max(e->ntypes, 1);
@@ -1030,13 +1047,15 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
       tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
     }
 
-  // exp(-k/32), k = 0 .. nexp-1, covering -(alpha rc_pair)^2 (at most 4096)
+  // exp(-k/64), k = 0 .. nexp-1, covering -(alpha rc_pair)^2 (at most 4096)
   std::vector<double> etab;
   if (common && pp.charge_style == CASS_Q_EWALD) {
     const double xm = pp.alpha_ewald * w.rc_pair;
-    const int ne = (int)std::min(4096.0, std::ceil(32.0 * xm * xm) + 2.0);
-    etab.resize(ne);
-    for (int k = 0; k < ne; ++k) etab[k] = std::exp(-k / 32.0);
+    const double need = std::ceil(64.0 * xm * xm) + 2.0;
+    if (need <= 4096.0) {  // otherwise the kernel takes the library exp
+      etab.resize((size_t)need);
+      for (size_t k = 0; k < etab.size(); ++k) etab[k] = std::exp(-(double)k / 64.0);
+    }
   }
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;
@@ -1073,6 +1092,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const size_t o_fm = carve(sizeof(FrameMol) * fm.size());
   const size_t o_fs = carve(sizeof(double4) * fs.size());
   const size_t o_ft = carve(sizeof(int) * ft.size());
+  const size_t o_fc = carve(sizeof(int) * fcode.size());
   const size_t o_tab = carve(sizeof(double4) * tab.size());
   const size_t o_tab2 = carve(sizeof(double4) * tab2.size());
   const size_t o_etab = carve(sizeof(double) * etab.size());
@@ -1095,6 +1115,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   FrameMol *d_fm = (FrameMol *)(s + o_fm);
   double4 *d_fs = (double4 *)(s + o_fs);
   int *d_ft = (int *)(s + o_ft);
+  int *d_fc = (int *)(s + o_fc);
   double4 *d_tab = (double4 *)(s + o_tab);
   double4 *d_tab2 = (double4 *)(s + o_tab2);
   double *d_etab = (double *)(s + o_etab);
@@ -1109,6 +1130,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     HIP_TRY(e, hipMemcpyAsync(d_fm, fm.data(), sizeof(FrameMol) * fm.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_fs, fs.data(), sizeof(double4) * fs.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_ft, ft.data(), sizeof(int) * ft.size(), hipMemcpyHostToDevice, e->stream));
+    HIP_TRY(e, hipMemcpyAsync(d_fc, fcode.data(), sizeof(int) * fcode.size(), hipMemcpyHostToDevice, e->stream));
   }
   HIP_TRY(e, hipMemcpyAsync(d_tab, tab.data(), sizeof(double4) * tab.size(), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipMemcpyAsync(d_tab2, tab2.data(), sizeof(double4) * tab2.size(), hipMemcpyHostToDevice, e->stream));
@@ -1190,7 +1212,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
                                d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
     HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
     const size_t shbytes = sizeof(double) * std::max<size_t>(etab.size(), 1);
-    hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_ft,
+    hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_fc,
                        pos, live_sites, d_tab2, d_q, d_etab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
