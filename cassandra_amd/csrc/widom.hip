@@ -353,8 +353,9 @@ __global__ void __launch_bounds__(256) k_widom_place(WidomArgs w, BoxDev b, int 
 struct RealArgs {
   int64_t n_ins, npad;  // insertions; slots per site kind (multiple of 64)
   int ns, ntypes, nmols, q_style;  // q_style: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT
-  int coul_cbmc, nexp, same_cut;   // same_cut: rc_vdw = rc_coul = rc_pair
+  int coul_cbmc, ntab, same_cut;   // same_cut: rc_vdw = rc_coul = rc_pair
   double rc_vdw2, rc_coul2, rc_cbmc2, rc_low2, rc_pair2, rc_pair, alpha;
+  double tab_inv_h;  // intervals per Angstrom of the erfc table
   double L[3], hL[3], Li[3];
   double dcom_max;  // largest max_dcom of the frame molecules
 };
@@ -375,20 +376,6 @@ __device__ inline double fma_v(double a, double b, double c) {
   double r;
   asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
   return r;
-}
-
-// exp(z) for -(alpha rc_pair)^2 <= z <= 0: exp(-k/64) from the table times a
-// degree-4 Taylor factor of the remainder |d| <= 1/128 (relative error
-// < 3e-13).  Lanes outside the cutoff, whose value is discarded, are clamped
-// to the table.
-__device__ inline double exp_tab(double z, const double *t, int n) {
-  const double kf = rint(-64.0 * z);
-  const double d = fma(kf, 0.015625, z);
-  double p = fma_v(d, 1.0 / 24.0, 1.0 / 6.0);
-  p = fma_v(d, p, 0.5);
-  p = fma_v(d, p, 1.0);
-  p = fma_v(d, p, 1.0);
-  return t[min((int)kf, n - 1)] * p;
 }
 
 // The pair terms of the real-space site kernel in the precision mode's
@@ -414,10 +401,8 @@ __device__ inline void site_pair_lj(double r2, double4 prm, double &e_v) {
 
 // Ewald real space (A&S erfc) or plain Coulomb.
 template <typename P, int QS>
-__device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, double qqK, const double *etab,
+__device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, double qqK, const double4 *qtab,
                                     double &e_q) {
-  const double A1 = 0.254829592, A2 = -0.284496736, A3 = 1.421413741, A4 = -1.453152027, A5 = 1.061405429,
-               PA = 0.3275911;
   if constexpr (std::is_same<P, PrecFP64>::value) {
     // the reference's operations (ewald_real / coul_cut with MathRef)
     const double rij = sqrt(r2);
@@ -429,27 +414,30 @@ __device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, do
       e_q = kChargeFactor * qq / rij;
     }
   } else if constexpr (std::is_same<P, PrecMixed>::value) {
-    // v_rsq_f64 / v_rcp_f64 refined by one Newton step, exp from the table
+    // v_rsq_f64 refined by one Newton step; the A&S erfc(alpha r) from its
+    // cubic Hermite table (interval i covers i/inv_h <= r < (i+1)/inv_h).
+    // Lanes beyond the table, whose value is discarded, read its last
+    // interval.
     double ri = __builtin_amdgcn_rsq(r2);
     ri = fma(0.5 * ri, fma(-r2 * ri, ri, 1.0), ri);
     if (QS == CASS_Q_EWALD) {
-      const double x = ra.alpha * (r2 * ri);
-      const double den = fma(PA, x, 1.0);
-      double t = __builtin_amdgcn_rcp(den);
-      t = fma(t, fma(-den, t, 1.0), t);
-      const double tp = t * fma_v(t, fma_v(t, fma_v(t, fma_v(t, A5, A4), A3), A2), A1);
-      const double ex = ra.nexp > 0 ? exp_tab(-(x * x), etab, ra.nexp) : exp(-(x * x));
-      e_q = qqK * ri * (tp * ex);
+      const double u = (r2 * ri) * ra.tab_inv_h;
+      const double4 c = qtab[(int)fmin(u, (double)(ra.ntab - 1))];
+      const double d = __builtin_amdgcn_fract(u);
+      const double f = fma_v(d, fma_v(d, fma_v(d, c.w, c.z), c.y), c.x);
+      e_q = qqK * ri * f;
     } else {
       e_q = qqK * ri;
     }
   } else {
+    const float A1 = 0.254829592f, A2 = -0.284496736f, A3 = 1.421413741f, A4 = -1.453152027f, A5 = 1.061405429f,
+                PA = 0.3275911f;
     const float r2f = (float)r2;
     const float ri = __builtin_amdgcn_rsqf(r2f);
     if (QS == CASS_Q_EWALD) {
       const float x = (float)ra.alpha * (r2f * ri);
-      const float t = __builtin_amdgcn_rcpf(fmaf((float)PA, x, 1.0f));
-      const float tp = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, (float)A5, (float)A4), (float)A3), (float)A2), (float)A1);
+      const float t = __builtin_amdgcn_rcpf(fmaf(PA, x, 1.0f));
+      const float tp = t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, A5, A4), A3), A2), A1);
       e_q = (double)((float)(qq * kChargeFactor) * ri * (tp * __expf(-(x * x))));
     } else {
       e_q = (double)((float)(kChargeFactor * qq) * ri);
@@ -457,40 +445,57 @@ __device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, do
   }
 }
 
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
 // Pair terms of one site (lane) against frame molecule m.  The wave holds
 // sites of one kind (qa and the LJ row are uniform), so the choice of terms
 // per frame site is a scalar branch.  The sites reaching this kernel have no
 // hard overlap (k_widom_overlap removed those insertions), so every lane
-// computes the pair and the cutoff tests select what is added: no per-lane
-// branches.  QS: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT; SAME: the vdW and
-// Coulomb cutoffs both equal rc_pair.  UNI: every site of m takes the image
-// shift of m's centre seen from the wave centre c, S = -L rint((c - com)/L);
-// when (wave extent + cutoff + max_dcom) is below half the box, S is the
-// per-pair minimum image of minimum_image_separation.f90:35-60 for every pair
-// inside the cutoff, and any other pair stays outside the cutoff, so
-// (p - s) + S is the reference's own separation, bit for bit.
+// computes the pair and the cutoff tests weight what is added (0 or 1): no
+// per-lane branches.  QS: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT; SAME: the
+// vdW and Coulomb cutoffs both equal rc_pair.  UNI: every site of m takes the
+// image shift of m's centre seen from the wave centre c,
+// S = -L rint((c - com)/L); when (wave extent + cutoff + max_dcom) is below
+// half the box, S is the per-pair minimum image of
+// minimum_image_separation.f90:35-60 for every pair inside the cutoff, and
+// any other pair stays outside the cutoff.  FP64 forms (p - s) + S, the
+// reference's own separation bit for bit; the other modes shift the lane's
+// site once per molecule, (p + S) - s.  cq: 1 when the CBMC energy includes
+// the Coulomb term, else 0.
 template <typename P, int QS, bool SAME, bool UNI>
 __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
-                                  const int *__restrict__ ftype, const double4 *__restrict__ lj_row,
-                                  const double *etab, double cx, double cy, double cz, bool live,
-                                  uint64_t live_mask, double qa, double qaK, bool qa_nz, double px, double py,
-                                  double pz,
-                                  double &ev, double &eq, double &ecb) {
+                                  const int *__restrict__ fcode, const double4 *__restrict__ lj_row,
+                                  const double4 *qtab, double cx, double cy, double cz, bool live,
+                                  uint64_t live_mask, double qa, double qaK, bool qa_nz, double cq, double px,
+                                  double py, double pz, double &ev, double &eq, double &ecb) {
+  constexpr bool kExactSep = std::is_same<P, PrecFP64>::value;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   if (UNI) {
     sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
     sy = -ra.L[1] * rint((cy - m.com.y) * ra.Li[1]);
     sz = -ra.L[2] * rint((cz - m.com.z) * ra.Li[2]);
   }
+  double qx = px, qy = py, qz = pz;
+  if (UNI && !kExactSep) {
+    qx += sx;
+    qy += sy;
+    qz += sz;
+  }
   for (int k = m.first; k < m.first + m.n; ++k) {
     const double4 st = fsite[k];
-    const int code = ftype[k];  // (atom type + 1) | charged << 16
+    const int code = fcode[k];  // (atom type + 1) | charged << 16
     const int tj = (code & 0xffff) - 1;
-    double dx = px - st.x, dy = py - st.y, dz = pz - st.z;
+    double dx = qx - st.x, dy = qy - st.y, dz = qz - st.z;
     if (UNI) {
-      dx += sx;
-      dy += sy;
-      dz += sz;
+      if (kExactSep) {
+        dx += sx;
+        dy += sy;
+        dz += sz;
+      }
     } else {
       dx = fma(-rint(dx * ra.Li[0]), ra.L[0], dx);
       dy = fma(-rint(dy * ra.Li[1]), ra.L[1], dy);
@@ -504,41 +509,43 @@ __device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const d
     const double4 prm = tj >= 0 ? lj_row[tj] : make_double4(0.0, 0.0, 0.0, 0.0);
     const bool has_v = prm.x != 0.0;  // eps = 0 (or no type) adds exactly 0
     const bool has_q = QS != CASS_Q_NONE && qa_nz && (code >> 16) != 0;
-    const double qq = qa * st.w;
     double e_v = 0.0, e_q = 0.0;
     if (has_v) site_pair_lj<P>(r2, prm, e_v);
-    if (has_q) site_pair_qq<P, QS>(ra, r2, qq, qaK * st.w, etab, e_q);
-    const bool incb = near && r2 <= ra.rc_cbmc2;
+    if (has_q) site_pair_qq<P, QS>(ra, r2, qa * st.w, qaK * st.w, qtab, e_q);
     if (SAME) {
-      ev += near ? e_v : 0.0;
-      eq += near ? e_q : 0.0;
+      const double w = near ? 1.0 : 0.0;
+      ev = fma(w, e_v, ev);
+      eq = fma(w, e_q, eq);
     } else {
-      ev += (near && r2 <= ra.rc_vdw2) ? e_v : 0.0;
-      eq += (near && r2 <= ra.rc_coul2) ? e_q : 0.0;
+      ev = fma((near && r2 <= ra.rc_vdw2) ? 1.0 : 0.0, e_v, ev);
+      eq = fma((near && r2 <= ra.rc_coul2) ? 1.0 : 0.0, e_q, eq);
     }
-    ecb += incb ? e_v + (ra.coul_cbmc ? e_q : 0.0) : 0.0;
+    ecb = fma((near && r2 <= ra.rc_cbmc2) ? 1.0 : 0.0, fma(cq, e_q, e_v), ecb);
   }
 }
 
 // One lane per site of the surviving insertions.  Slots are laid out kind by
 // kind, npad (a multiple of 64) per kind, so every wavefront holds sites of a
-// single kind.
+// single kind.  Mixed precision with Ewald reads the erfc table (ra.ntab
+// double4 intervals) from LDS.
 template <typename P, int QS, bool SAME>
 __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const FrameMol *__restrict__ fmol,
                                                         const double4 *__restrict__ fsite,
-                                                        const int *__restrict__ ftype, const double *__restrict__ pos,
+                                                        const int *__restrict__ fcode, const double *__restrict__ pos,
                                                         const int *__restrict__ order, const double4 *__restrict__ ptab,
                                                         const double *__restrict__ site_q,
-                                                        const double *__restrict__ etab_g,
+                                                        const double4 *__restrict__ qtab_g,
                                                         const int *__restrict__ nlive_sites,
                                                         double4 *__restrict__ out) {
   // surviving insertions: m per kind, npad = m rounded up to whole waves
   const int64_t m = *nlive_sites / ra.ns;
   const int64_t npad = (m + 63) / 64 * 64;
   if ((int64_t)blockIdx.x * kWBlock >= (int64_t)ra.ns * npad) return;
-  extern __shared__ double etab[];  // exp(-k/64)
-  for (int t = threadIdx.x; t < ra.nexp; t += blockDim.x) etab[t] = etab_g[t];
-  __syncthreads();
+  extern __shared__ double4 qtab[];
+  if constexpr (std::is_same<P, PrecMixed>::value && QS == CASS_Q_EWALD) {
+    for (int t = threadIdx.x; t < ra.ntab; t += blockDim.x) qtab[t] = qtab_g[t];
+    __syncthreads();
+  }
   const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
   const int a = __builtin_amdgcn_readfirstlane((int)min(s / npad, (int64_t)ra.ns - 1));
   const int64_t idx = s - (int64_t)a * npad;
@@ -548,13 +555,26 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   const double qa = site_q[a];
   const bool qa_nz = qa != 0.0;
   const double qaK = qa * kChargeFactor;
+  const double cq = ra.coul_cbmc ? 1.0 : 0.0;
   const uint64_t live_mask = __builtin_amdgcn_ballot_w64(active);
   const double4 *lj_row = ptab + (int64_t)a * ra.ntypes;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
   double ev = 0.0, eq = 0.0, ecb = 0.0;
-  // wave centre (first lane's site) and the wave's extent around it
-  const double cx = wave_uniform(px), cy = wave_uniform(py), cz = wave_uniform(pz);
-  double dx = px - cx, dy = py - cy, dz = pz - cz;
+  // wave centre: the mean of the wave's sites, taken as minimum images
+  // around the first lane's site (inactive lanes repeat a site of the wave)
+  const double x0 = wave_uniform(px), y0 = wave_uniform(py), z0 = wave_uniform(pz);
+  double dx = px - x0, dy = py - y0, dz = pz - z0;
+  dx -= ra.L[0] * rint(dx * ra.Li[0]);
+  dy -= ra.L[1] * rint(dy * ra.Li[1]);
+  dz -= ra.L[2] * rint(dz * ra.Li[2]);
+  const double cx = x0 + wave_uniform(wave_sum(dx)) * (1.0 / 64.0);
+  const double cy = y0 + wave_uniform(wave_sum(dy)) * (1.0 / 64.0);
+  const double cz = z0 + wave_uniform(wave_sum(dz)) * (1.0 / 64.0);
+  // the wave's extent around c (coordinates as stored) and its radius
+  // (minimum image)
+  dx = px - cx;
+  dy = py - cy;
+  dz = pz - cz;
   const double ext = wave_max(fmax(fabs(dx), fmax(fabs(dy), fabs(dz))));
   dx -= ra.L[0] * rint(dx * ra.Li[0]);
   dy -= ra.L[1] * rint(dy * ra.Li[1]);
@@ -579,11 +599,11 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
       mask &= mask - 1;
       const FrameMol fm = fmol[j];
       if (uni)
-        site_pairs<P, QS, SAME, true>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, live_mask, qa, qaK, qa_nz,
-                                      px, py, pz, ev, eq, ecb);
+        site_pairs<P, QS, SAME, true>(ra, fm, fsite, fcode, lj_row, qtab, cx, cy, cz, active, live_mask, qa, qaK,
+                                      qa_nz, cq, px, py, pz, ev, eq, ecb);
       else
-        site_pairs<P, QS, SAME, false>(ra, fm, fsite, ftype, lj_row, etab, cx, cy, cz, active, live_mask, qa,
-                                       qaK, qa_nz, px, py, pz, ev, eq, ecb);
+        site_pairs<P, QS, SAME, false>(ra, fm, fsite, fcode, lj_row, qtab, cx, cy, cz, active, live_mask, qa, qaK,
+                                       qa_nz, cq, px, py, pz, ev, eq, ecb);
     }
   }
   if (active) out[v] = make_double4(ev, eq, ecb, 0.0);
@@ -908,6 +928,18 @@ static bool bv_cubic(const cass_box_state &b) { return b.p.box_shape == CASS_BOX
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// The A&S 7.1.26 erfc of energy_routines.f90:2143-2163 at x = al r, and its
+// derivative in r, in extended precision (the nodes of the cubic table).
+static void erfc_as_host(long double x, long double al, long double &f, long double &dfdr) {
+  const long double A1 = 0.254829592L, A2 = -0.284496736L, A3 = 1.421413741L, A4 = -1.453152027L,
+                    A5 = 1.061405429L, PA = 0.3275911L;
+  const long double t = 1.0L / (1.0L + PA * x), ex = expl(-x * x);
+  const long double q = t * (A1 + t * (A2 + t * (A3 + t * (A4 + t * A5))));
+  const long double dq = A1 + t * (2 * A2 + t * (3 * A3 + t * (4 * A4 + t * 5 * A5)));
+  f = q * ex;
+  dfdr = al * (dq * (-PA * t * t) - 2.0L * x * q) * ex;
+}
+
 // Cell grid of the insertion sort: about 64 insertions per cell, at most
 // 64 cells per box edge.
 static void widom_grid(const cass_box_state &b, int64_t n, int g[3]) {
@@ -924,7 +956,7 @@ typedef void (*recip_kernel_t)(WidomArgs, BoxDev, EwaldDev, const double4 *, con
 typedef void (*finish_kernel_t)(WidomArgs, const int *, const double4 *, const double *, double *, int64_t *,
                                 double *, double *);
 typedef void (*real_kernel_t)(RealArgs, const FrameMol *, const double4 *, const int *, const double *, const int *,
-                              const double4 *, const double *, const double *, const int *, double4 *);
+                              const double4 *, const double *, const double4 *, const int *, double4 *);
 
 template <int NS>
 static void split_kernels(place_kernel_t &pl, recip_kernel_t &rc, finish_kernel_t &fi) {
@@ -1047,14 +1079,22 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
       tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
     }
 
-  // exp(-k/64), k = 0 .. nexp-1, covering -(alpha rc_pair)^2 (at most 4096)
-  std::vector<double> etab;
-  if (common && pp.charge_style == CASS_Q_EWALD) {
+  // Mixed precision, Ewald: cubic Hermite table of the A&S erfc(alpha r)
+  // over 0 <= r <= rc_pair (ntab intervals, rc_pair at u = ntab - 2)
+  std::vector<double4> qtab;
+  double tab_inv_h = 0.0;
+  if (common && pp.charge_style == CASS_Q_EWALD && e->prec == CASS_PREC_MIXED) {
     const double xm = pp.alpha_ewald * w.rc_pair;
-    const double need = std::ceil(64.0 * xm * xm) + 2.0;
-    if (need <= 4096.0) {  // otherwise the kernel takes the library exp
-      etab.resize((size_t)need);
-      for (size_t k = 0; k < etab.size(); ++k) etab[k] = std::exp(-(double)k / 64.0);
+    const int nt_q = (int)std::min(1024.0, std::max(64.0, std::ceil(xm / 0.004) + 2.0));
+    tab_inv_h = (nt_q - 2) / w.rc_pair;
+    const long double h = 1.0L / tab_inv_h, al = pp.alpha_ewald;
+    std::vector<long double> f(nt_q + 1), df(nt_q + 1);
+    for (int i = 0; i <= nt_q; ++i) erfc_as_host(al * h * i, al, f[i], df[i]);
+    qtab.resize(nt_q);
+    for (int i = 0; i < nt_q; ++i) {
+      const long double f0 = f[i], f1 = f[i + 1], d0 = h * df[i], d1 = h * df[i + 1];
+      qtab[i] = make_double4((double)f0, (double)d0, (double)(3 * (f1 - f0) - 2 * d0 - d1),
+                             (double)(2 * (f0 - f1) + d0 + d1));
     }
   }
   const size_t n = (size_t)n_ins;
@@ -1095,7 +1135,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const size_t o_fc = carve(sizeof(int) * fcode.size());
   const size_t o_tab = carve(sizeof(double4) * tab.size());
   const size_t o_tab2 = carve(sizeof(double4) * tab2.size());
-  const size_t o_etab = carve(sizeof(double) * etab.size());
+  const size_t o_qtab = carve(sizeof(double4) * qtab.size());
   const size_t o_q = carve(sizeof(double) * kMaxWidomSites);
   const size_t o_pos = carve(common && !place ? sizeof(double) * 3 * n * ns : 0);
   const size_t o_sp = carve(common ? sizeof(double4) * n * ns : 0);
@@ -1118,7 +1158,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   int *d_fc = (int *)(s + o_fc);
   double4 *d_tab = (double4 *)(s + o_tab);
   double4 *d_tab2 = (double4 *)(s + o_tab2);
-  double *d_etab = (double *)(s + o_etab);
+  double4 *d_qtab = (double4 *)(s + o_qtab);
   double *d_q = (double *)(s + o_q);
   double *d_pos = (double *)(s + o_pos);
   double4 *d_sp = (double4 *)(s + o_sp);
@@ -1134,8 +1174,8 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   HIP_TRY(e, hipMemcpyAsync(d_tab, tab.data(), sizeof(double4) * tab.size(), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipMemcpyAsync(d_tab2, tab2.data(), sizeof(double4) * tab2.size(), hipMemcpyHostToDevice, e->stream));
-  if (!etab.empty())
-    HIP_TRY(e, hipMemcpyAsync(d_etab, etab.data(), sizeof(double) * etab.size(), hipMemcpyHostToDevice, e->stream));
+  if (!qtab.empty())
+    HIP_TRY(e, hipMemcpyAsync(d_qtab, qtab.data(), sizeof(double4) * qtab.size(), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipMemcpyAsync(d_q, w.q, sizeof(double) * kMaxWidomSites, hipMemcpyHostToDevice, e->stream));
 
   double *d_place = nullptr, *d_var = nullptr, *d_eout = nullptr;
@@ -1183,7 +1223,8 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     ra.nmols = (int)fm.size();
     ra.q_style = pp.charge_style;
     ra.coul_cbmc = coul_cbmc;
-    ra.nexp = (int)etab.size();
+    ra.ntab = (int)qtab.size();
+    ra.tab_inv_h = tab_inv_h;
     ra.rc_vdw2 = pp.vdw_style == CASS_VDW_NONE ? -1.0 : pp.rcut_vdwsq;
     ra.rc_coul2 = pp.charge_style == CASS_Q_NONE ? -1.0 : pp.rcut_coulsq;
     ra.rc_cbmc2 = pp.rcut_cbmcsq;
@@ -1211,9 +1252,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     HIP_TRY(e, rocprim::select((void *)(s + o_tmp), sort_tmp, rocprim::counting_iterator<int>(0), live_ins,
                                d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
     HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
-    const size_t shbytes = sizeof(double) * std::max<size_t>(etab.size(), 1);
+    const size_t shbytes = sizeof(double4) * std::max<size_t>(qtab.size(), 1);
     hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_fc,
-                       pos, live_sites, d_tab2, d_q, d_etab, d_cnt, d_sp);
+                       pos, live_sites, d_tab2, d_q, d_qtab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
     if (w.charged && ew.nvecs > 0) {
