@@ -138,9 +138,10 @@ __global__ void k_widom_weights(EwaldDev ew, double4 *__restrict__ wk) {
 // Reciprocal-space energy change of adding the test molecule:
 // charge_factor * sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2), written as
 // sum_k [Cn (dC^2 + dS^2) + (2 Cn C) dC + (2 Cn S) dS], with
-// q_a exp(i h.r_a) = q_a Ea^nx Eb^ny Ec^nz built by phase recurrence along
-// the k rows (h.r = 2 pi n.s, s the fractional coordinates, for the k sets of
-// both branches of Ewald_Reciprocal_Lattice_Vector_Setup).  FP64 throughout.
+// q_a exp(i h.r_a) = q_a Ea^nx Eb^ny Ec^nz built by recurrences along the k
+// rows (h.r = 2 pi n.s, s the fractional coordinates, for the k sets of both
+// branches of Ewald_Reciprocal_Lattice_Vector_Setup): complex products across
+// rows, the two-term cosine recurrence within a row.  FP64 throughout.
 template <int NS>
 __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__restrict__ wk, const BoxDev &b,
                                        const WidomArgs &w, const double (&px)[NS], const double (&py)[NS],
@@ -161,6 +162,9 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__rest
     Pb[a] = {1.0, 0.0};
     Pc[a] = {1.0, 0.0};
   }
+  double C2[NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a) C2[a] = 2.0 * Ea[a].re;
   int cur_ny = -1000000, cur_nz = -1000000;
   double de = 0.0;
   for (int r = 0; r < ew.nrows; ++r) {
@@ -180,12 +184,15 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__rest
       for (int a = 0; a < NS; ++a) Pb[a] = cpow_int(Eb[a], ny);
     }
     cur_ny = ny;
-    cplx T[NS];
+    // T: the site's term at nx, U: at nx - 1; along the row
+    // T(nx + 1) = 2 cos(theta) T(nx) - T(nx - 1), theta = 2 pi s_x
+    cplx T[NS], U[NS];
 #pragma unroll
     for (int a = 0; a < NS; ++a) {
       cplx base = cmul(Pb[a], Pc[a]);
       if (nx0 != 0) base = cmul(base, cpow_int(Ea[a], nx0));
       T[a] = {w.q[a] * base.re, w.q[a] * base.im};
+      U[a] = cmul(T[a], {Ea[a].re, -Ea[a].im});
     }
     const double4 *wr = wk + ew.row_k0[r];
 #pragma unroll 2
@@ -197,7 +204,11 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__rest
         ds += T[a].im;
       }
 #pragma unroll
-      for (int a = 0; a < NS; ++a) T[a] = cmul(T[a], Ea[a]);
+      for (int a = 0; a < NS; ++a) {
+        const cplx nx = {fma(C2[a], T[a].re, -U[a].re), fma(C2[a], T[a].im, -U[a].im)};
+        U[a] = T[a];
+        T[a] = nx;
+      }
       const double4 c = wr[kk];
       de = fma(c.x, fma(dc, dc, ds * ds), de);
       de = fma(c.y, dc, de);
@@ -355,7 +366,8 @@ struct RealArgs {
   int ns, ntypes, nmols, q_style;  // q_style: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT
   int coul_cbmc, ntab, same_cut;   // same_cut: rc_vdw = rc_coul = rc_pair
   double rc_vdw2, rc_coul2, rc_cbmc2, rc_low2, rc_pair2, rc_pair, alpha;
-  double tab_inv_h;  // intervals per Angstrom of the erfc table
+  double tab_inv_h;     // intervals per Angstrom of the erfc table
+  int64_t psite_stride;  // frame-site records per kind in the parameter table
   double L[3], hL[3], Li[3];
   double dcom_max;  // largest max_dcom of the frame molecules
 };
@@ -422,7 +434,7 @@ __device__ inline void site_pair_qq(const RealArgs &ra, double r2, double qq, do
     ri = fma(0.5 * ri, fma(-r2 * ri, ri, 1.0), ri);
     if (QS == CASS_Q_EWALD) {
       const double u = (r2 * ri) * ra.tab_inv_h;
-      const double4 c = qtab[(int)fmin(u, (double)(ra.ntab - 1))];
+      const double4 c = qtab[min((int)u, ra.ntab - 1)];  // u < 2^31: r is within a few box lengths
       const double d = __builtin_amdgcn_fract(u);
       const double f = fma_v(d, fma_v(d, fma_v(d, c.w, c.z), c.y), c.x);
       e_q = qqK * ri * f;
@@ -451,89 +463,101 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// Pair terms of one site (lane) against frame molecule m.  The wave holds
-// sites of one kind (qa and the LJ row are uniform), so the choice of terms
-// per frame site is a scalar branch.  The sites reaching this kernel have no
-// hard overlap (k_widom_overlap removed those insertions), so every lane
-// computes the pair and the cutoff tests weight what is added (0 or 1): no
-// per-lane branches.  QS: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT; SAME: the
-// vdW and Coulomb cutoffs both equal rc_pair.  UNI: every site of m takes the
-// image shift of m's centre seen from the wave centre c,
-// S = -L rint((c - com)/L); when (wave extent + cutoff + max_dcom) is below
-// half the box, S is the per-pair minimum image of
+// Bits of a wave-uniform double other than its sign: nonzero unless v = +0
+// or -0.  The empty asm keeps the test on the scalar unit (as a floating
+// compare it would take a vector instruction).
+__device__ inline bool nonzero_bits(double v) {
+  uint64_t b = __builtin_bit_cast(uint64_t, v);
+  asm("" : "+s"(b));
+  return (b << 1) != 0ull;
+}
+
+// Pair terms of one site (lane) against the sites of one frame molecule:
+// positions ps[0 .. n) and the wave's per-site parameters pp[0 .. n)
+// (4 eps, sigma^2, shift, charge product), both readable 4 entries past n.
+// The wave holds sites of one kind, so the parameters are wave-uniform and
+// the choice of terms is a scalar branch.  The sites reaching this kernel
+// have no hard overlap (k_widom_overlap removed those insertions), so every
+// lane computes the pair and the cutoff tests weight what is added (0 or 1):
+// no per-lane branches.  QS: CASS_Q_NONE / CASS_Q_EWALD / CASS_Q_CUT; SAME:
+// the vdW and Coulomb cutoffs both equal rc_pair.  UNI: every site of the
+// molecule takes the image shift sh of its centre seen from the wave centre
+// c, S = -L rint((c - com)/L); when (wave extent + cutoff + max_dcom) is
+// below half the box, S is the per-pair minimum image of
 // minimum_image_separation.f90:35-60 for every pair inside the cutoff, and
 // any other pair stays outside the cutoff.  FP64 forms (p - s) + S, the
 // reference's own separation bit for bit; the other modes shift the lane's
-// site once per molecule, (p + S) - s.  cq: 1 when the CBMC energy includes
-// the Coulomb term, else 0.
+// site once per molecule, (p + S) - s.  coul_cbmc: the CBMC energy includes
+// the Coulomb term.  Lanes with live false (copies of a live site of
+// the same wave) compute but add nothing.
 template <typename P, int QS, bool SAME, bool UNI>
-__device__ inline void site_pairs(const RealArgs &ra, const FrameMol &m, const double4 *__restrict__ fsite,
-                                  const int *__restrict__ fcode, const double4 *__restrict__ lj_row,
-                                  const double4 *qtab, double cx, double cy, double cz, bool live,
-                                  uint64_t live_mask, double qa, double qaK, bool qa_nz, double cq, double px,
-                                  double py, double pz, double &ev, double &eq, double &ecb) {
+__device__ inline void mol_pairs(const RealArgs &ra, const double4 *__restrict__ ps,
+                                 const double4 *__restrict__ pp, int n, double4 sh, const double4 *qtab, bool live,
+                                 bool coul_cbmc, double px, double py, double pz, double &ev, double &eq, double &ecb) {
   constexpr bool kExactSep = std::is_same<P, PrecFP64>::value;
-  double sx = 0.0, sy = 0.0, sz = 0.0;
-  if (UNI) {
-    sx = -ra.L[0] * rint((cx - m.com.x) * ra.Li[0]);
-    sy = -ra.L[1] * rint((cy - m.com.y) * ra.Li[1]);
-    sz = -ra.L[2] * rint((cz - m.com.z) * ra.Li[2]);
-  }
   double qx = px, qy = py, qz = pz;
   if (UNI && !kExactSep) {
-    qx += sx;
-    qy += sy;
-    qz += sz;
+    qx += sh.x;
+    qy += sh.y;
+    qz += sh.z;
   }
-  for (int k = m.first; k < m.first + m.n; ++k) {
-    const double4 st = fsite[k];
-    const int code = fcode[k];  // (atom type + 1) | charged << 16
-    const int tj = (code & 0xffff) - 1;
-    double dx = qx - st.x, dy = qy - st.y, dz = qz - st.z;
-    if (UNI) {
-      if (kExactSep) {
-        dx += sx;
-        dy += sy;
-        dz += sz;
+  for (int i = 0; i < n; i += 4) {
+    double4 st[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) st[u] = ps[i + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (i + u >= n) break;
+      double dx = qx - st[u].x, dy = qy - st[u].y, dz = qz - st[u].z;
+      if (UNI) {
+        if (kExactSep) {
+          dx += sh.x;
+          dy += sh.y;
+          dz += sh.z;
+        }
+      } else {
+        dx = fma(-rint(dx * ra.Li[0]), ra.L[0], dx);
+        dy = fma(-rint(dy * ra.Li[1]), ra.L[1], dy);
+        dz = fma(-rint(dz * ra.Li[2]), ra.L[2], dz);
       }
-    } else {
-      dx = fma(-rint(dx * ra.Li[0]), ra.L[0], dx);
-      dy = fma(-rint(dy * ra.Li[1]), ra.L[1], dy);
-      dz = fma(-rint(dz * ra.Li[2]), ra.L[2], dz);
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      const bool inr = r2 <= ra.rc_pair2;
+      // a lane with live false repeats a live lane's site: no need to mask it
+      if (__builtin_amdgcn_ballot_w64(inr) == 0ull) continue;
+      const bool near = inr && live;
+      const double4 prm = pp[i + u];
+      const double wc = (near && r2 <= ra.rc_cbmc2) ? 1.0 : 0.0;
+      const double wn = near ? 1.0 : 0.0;
+      if (nonzero_bits(prm.x)) {  // 4 eps
+        double e_v;
+        site_pair_lj<P>(r2, prm, e_v);
+        ev = fma(SAME ? wn : ((near && r2 <= ra.rc_vdw2) ? 1.0 : 0.0), e_v, ev);
+        ecb = fma(wc, e_v, ecb);
+      }
+      if (QS != CASS_Q_NONE && nonzero_bits(prm.w)) {  // charge product
+        double e_q;
+        site_pair_qq<P, QS>(ra, r2, prm.w, prm.w, qtab, e_q);
+        eq = fma(SAME ? wn : ((near && r2 <= ra.rc_coul2) ? 1.0 : 0.0), e_q, eq);
+        if (coul_cbmc) ecb = fma(wc, e_q, ecb);
+      }
     }
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    const bool inr = r2 <= ra.rc_pair2;
-    if ((__builtin_amdgcn_ballot_w64(inr) & live_mask) == 0ull) continue;
-    const bool near = inr && live;
-    // uniform per frame site: the LJ row entry and whether the pair is charged
-    const double4 prm = tj >= 0 ? lj_row[tj] : make_double4(0.0, 0.0, 0.0, 0.0);
-    const bool has_v = prm.x != 0.0;  // eps = 0 (or no type) adds exactly 0
-    const bool has_q = QS != CASS_Q_NONE && qa_nz && (code >> 16) != 0;
-    double e_v = 0.0, e_q = 0.0;
-    if (has_v) site_pair_lj<P>(r2, prm, e_v);
-    if (has_q) site_pair_qq<P, QS>(ra, r2, qa * st.w, qaK * st.w, qtab, e_q);
-    if (SAME) {
-      const double w = near ? 1.0 : 0.0;
-      ev = fma(w, e_v, ev);
-      eq = fma(w, e_q, eq);
-    } else {
-      ev = fma((near && r2 <= ra.rc_vdw2) ? 1.0 : 0.0, e_v, ev);
-      eq = fma((near && r2 <= ra.rc_coul2) ? 1.0 : 0.0, e_q, eq);
-    }
-    ecb = fma((near && r2 <= ra.rc_cbmc2) ? 1.0 : 0.0, fma(cq, e_q, e_v), ecb);
   }
 }
 
+__device__ inline bool wave_uniform_bool(bool v) { return __builtin_amdgcn_readfirstlane(v ? 1 : 0) != 0; }
+
 // One lane per site of the surviving insertions.  Slots are laid out kind by
 // kind, npad (a multiple of 64) per kind, so every wavefront holds sites of a
-// single kind.  Mixed precision with Ewald reads the erfc table (ra.ntab
-// double4 intervals) from LDS.
+// single kind.  psite: per kind, one record per frame site (stride
+// ra.psite_stride); imgtab: the 27 image shifts (nx L, ny L, nz L, 0) at
+// (nx + 1) + 3 (ny + 1) + 9 (nz + 1).  Mixed precision with Ewald reads the
+// erfc table (ra.ntab double4 intervals) from LDS.
 template <typename P, int QS, bool SAME>
 __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const FrameMol *__restrict__ fmol,
                                                         const double4 *__restrict__ fsite,
-                                                        const int *__restrict__ fcode, const double *__restrict__ pos,
-                                                        const int *__restrict__ order, const double4 *__restrict__ ptab,
-                                                        const double *__restrict__ site_q,
+                                                        const double *__restrict__ pos, const int *__restrict__ order,
+                                                        const double4 *__restrict__ psite,
+                                                        const double4 *__restrict__ imgtab,
                                                         const double4 *__restrict__ qtab_g,
                                                         const int *__restrict__ nlive_sites,
                                                         double4 *__restrict__ out) {
@@ -552,12 +576,8 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   const bool active = s < (int64_t)ra.ns * npad && idx < m;
   if (!wave_any(active)) return;
   const int v = order[(int64_t)a * m + (active ? idx : m - 1)];
-  const double qa = site_q[a];
-  const bool qa_nz = qa != 0.0;
-  const double qaK = qa * kChargeFactor;
-  const double cq = ra.coul_cbmc ? 1.0 : 0.0;
-  const uint64_t live_mask = __builtin_amdgcn_ballot_w64(active);
-  const double4 *lj_row = ptab + (int64_t)a * ra.ntypes;
+  const bool coul_cbmc = ra.coul_cbmc != 0;
+  const double4 *psa = psite + (int64_t)a * ra.psite_stride;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
   double ev = 0.0, eq = 0.0, ecb = 0.0;
   // wave centre: the mean of the wave's sites, taken as minimum images
@@ -580,30 +600,40 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   dy -= ra.L[1] * rint(dy * ra.Li[1]);
   dz -= ra.L[2] * rint(dz * ra.Li[2]);
   const double rad = sqrt(wave_max(dx * dx + dy * dy + dz * dz));
-  const bool uni = ext + ra.rc_pair + ra.dcom_max + 1e-3 < fmin(ra.hL[0], fmin(ra.hL[1], ra.hL[2]));
+  const bool uni =
+      wave_uniform_bool(ext + ra.rc_pair + ra.dcom_max + 1e-3 < fmin(ra.hL[0], fmin(ra.hL[1], ra.hL[2])));
   for (int base = 0; base < ra.nmols; base += 64) {
     const int jt = base + (int)(threadIdx.x & 63);
     bool cand = false;
+    int code = 0;
     if (jt < ra.nmols) {
-      const double4 cj = fmol[jt].com;
-      double ex = cj.x - cx, ey = cj.y - cy, ez = cj.z - cz;
-      ex -= ra.L[0] * rint(ex * ra.Li[0]);
-      ey -= ra.L[1] * rint(ey * ra.Li[1]);
-      ez -= ra.L[2] * rint(ez * ra.Li[2]);
-      const double reach = ra.rc_pair + cj.w + rad + 1e-9;
+      const FrameMol fj = fmol[jt];
+      double ex = fj.com.x - cx, ey = fj.com.y - cy, ez = fj.com.z - cz;
+      const double nx = rint(ex * ra.Li[0]), ny = rint(ey * ra.Li[1]), nz = rint(ez * ra.Li[2]);
+      ex -= ra.L[0] * nx;
+      ey -= ra.L[1] * ny;
+      ez -= ra.L[2] * nz;
+      const double reach = ra.rc_pair + fj.com.w + rad + 1e-9;
       cand = ex * ex + ey * ey + ez * ez <= reach * reach;
+      // the shift S = -L rint((c - com)/L) = L rint((com - c)/L), one of
+      // -1, 0, 1 boxes per axis when uni holds; packed with the molecule's
+      // site range (n < 256, first < 2^19)
+      const int ix = min(1, max(-1, (int)nx)), iy = min(1, max(-1, (int)ny)), iz = min(1, max(-1, (int)nz));
+      const int img = (ix + 1) + 3 * (iy + 1) + 9 * (iz + 1);
+      code = (uni ? img : 13) | (fj.n << 5) | (fj.first << 13);
     }
     uint64_t mask = __builtin_amdgcn_ballot_w64(cand);
     while (mask) {
-      const int j = base + __builtin_ctzll(mask);
+      const int l = __builtin_ctzll(mask);
       mask &= mask - 1;
-      const FrameMol fm = fmol[j];
+      const int c = __builtin_amdgcn_readlane(code, l);
+      const int first = c >> 13, nsite = (c >> 5) & 255;
       if (uni)
-        site_pairs<P, QS, SAME, true>(ra, fm, fsite, fcode, lj_row, qtab, cx, cy, cz, active, live_mask, qa, qaK,
-                                      qa_nz, cq, px, py, pz, ev, eq, ecb);
+        mol_pairs<P, QS, SAME, true>(ra, fsite + first, psa + first, nsite, imgtab[c & 31], qtab, active, coul_cbmc, px,
+                                     py, pz, ev, eq, ecb);
       else
-        site_pairs<P, QS, SAME, false>(ra, fm, fsite, fcode, lj_row, qtab, cx, cy, cz, active, live_mask, qa, qaK,
-                                       qa_nz, cq, px, py, pz, ev, eq, ecb);
+        mol_pairs<P, QS, SAME, false>(ra, fsite + first, psa + first, nsite, make_double4(0.0, 0.0, 0.0, 0.0),
+                                      qtab, active, coul_cbmc, px, py, pz, ev, eq, ecb);
     }
   }
   if (active) out[v] = make_double4(ev, eq, ecb, 0.0);
@@ -955,8 +985,8 @@ typedef void (*recip_kernel_t)(WidomArgs, BoxDev, EwaldDev, const double4 *, con
                                const int *, double *);
 typedef void (*finish_kernel_t)(WidomArgs, const int *, const double4 *, const double *, double *, int64_t *,
                                 double *, double *);
-typedef void (*real_kernel_t)(RealArgs, const FrameMol *, const double4 *, const int *, const double *, const int *,
-                              const double4 *, const double *, const double4 *, const int *, double4 *);
+typedef void (*real_kernel_t)(RealArgs, const FrameMol *, const double4 *, const double *, const int *,
+                              const double4 *, const double4 *, const double4 *, const int *, double4 *);
 
 template <int NS>
 static void split_kernels(place_kernel_t &pl, recip_kernel_t &rc, finish_kernel_t &fi) {
@@ -1030,7 +1060,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   // the split path covers LJ cut / cut_tail / cut_shift with Ewald, cut
   // Coulomb or no charges; the fused kernel handles every other style
-  const bool common = (pp.vdw_style == CASS_VDW_NONE ||
+  bool common = (pp.vdw_style == CASS_VDW_NONE ||
                        (pp.vdw_style == CASS_VDW_LJ &&
                         (pp.vdw_sum_style == CASS_SUM_CUT || pp.vdw_sum_style == CASS_SUM_CUT_TAIL ||
                          pp.vdw_sum_style == CASS_SUM_CUT_SHIFT))) &&
@@ -1041,7 +1071,8 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   // frame tables (host mirrors -> device)
   std::vector<FrameMol> fm;
   std::vector<double4> fs;
-  std::vector<int> ft, fcode;
+  std::vector<int> ft;
+  int max_nat = 0;
   fm.reserve(b.nmols);
   for (int m = 0; m < b.nmols; ++m) {
     if (!b.live[m]) continue;
@@ -1056,9 +1087,14 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     for (int a = 0; a < nat; ++a) {
       fs.push_back(make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], e->atom_charge[off + a]));
       ft.push_back(e->atom_type[off + a]);
-      fcode.push_back((e->atom_type[off + a] + 1) | (e->atom_charge[off + a] != 0.0 ? 0x10000 : 0));
     }
+    max_nat = std::max(max_nat, nat);
   }
+  const size_t nfs = fs.size();
+  // the real-space kernel reads 4 frame sites past a molecule's last and
+  // packs (first site, site count) into 19 + 8 bits
+  for (int k = 0; k < 4; ++k) fs.push_back(make_double4(0.0, 0.0, 0.0, 0.0));
+  if (nfs >= ((size_t)1 << 19) || max_nat > 255) common = false;
   const int nt = std::max(e->ntypes, 1);
   std::vector<double4> tab((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
   // split-path LJ table: (4 eps, sigma^2, shift, 0); the shift is the
@@ -1078,6 +1114,25 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
       }
       tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
     }
+
+  // per probe kind, one record per frame site: the LJ row entry and the
+  // charge product q_a q_j (times charge_factor in mixed precision, the
+  // factor the pair function of that mode leaves out)
+  const int64_t ps_stride = (int64_t)fs.size();
+  std::vector<double4> psite(common ? (size_t)ns * ps_stride : 0, make_double4(0.0, 0.0, 0.0, 0.0));
+  if (common)
+    for (int a = 0; a < ns; ++a)
+      for (size_t f = 0; f < nfs; ++f) {
+        const int tj = ft[f];
+        double4 r = tj >= 0 ? tab2[(size_t)a * nt + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
+        r.w = e->prec == CASS_PREC_MIXED ? (w.q[a] * kChargeFactor) * fs[f].w : w.q[a] * fs[f].w;
+        psite[(size_t)a * ps_stride + f] = r;
+      }
+  // image shifts (nx L, ny L, nz L) at (nx + 1) + 3 (ny + 1) + 9 (nz + 1)
+  std::vector<double4> imgtab(27);
+  for (int k = 0; k < 27; ++k)
+    imgtab[k] = make_double4((k % 3 - 1) * pp.basis_length[0], (k / 3 % 3 - 1) * pp.basis_length[1],
+                             (k / 9 - 1) * pp.basis_length[2], 0.0);
 
   // Mixed precision, Ewald: cubic Hermite table of the A&S erfc(alpha r)
   // over 0 <= r <= rc_pair (ntab intervals, rc_pair at u = ntab - 2)
@@ -1132,9 +1187,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const size_t o_fm = carve(sizeof(FrameMol) * fm.size());
   const size_t o_fs = carve(sizeof(double4) * fs.size());
   const size_t o_ft = carve(sizeof(int) * ft.size());
-  const size_t o_fc = carve(sizeof(int) * fcode.size());
   const size_t o_tab = carve(sizeof(double4) * tab.size());
-  const size_t o_tab2 = carve(sizeof(double4) * tab2.size());
+  const size_t o_psite = carve(sizeof(double4) * psite.size());
+  const size_t o_img = carve(sizeof(double4) * imgtab.size());
   const size_t o_qtab = carve(sizeof(double4) * qtab.size());
   const size_t o_q = carve(sizeof(double) * kMaxWidomSites);
   const size_t o_pos = carve(common && !place ? sizeof(double) * 3 * n * ns : 0);
@@ -1155,9 +1210,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   FrameMol *d_fm = (FrameMol *)(s + o_fm);
   double4 *d_fs = (double4 *)(s + o_fs);
   int *d_ft = (int *)(s + o_ft);
-  int *d_fc = (int *)(s + o_fc);
   double4 *d_tab = (double4 *)(s + o_tab);
-  double4 *d_tab2 = (double4 *)(s + o_tab2);
+  double4 *d_psite = (double4 *)(s + o_psite);
+  double4 *d_img = (double4 *)(s + o_img);
   double4 *d_qtab = (double4 *)(s + o_qtab);
   double *d_q = (double *)(s + o_q);
   double *d_pos = (double *)(s + o_pos);
@@ -1170,10 +1225,11 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     HIP_TRY(e, hipMemcpyAsync(d_fm, fm.data(), sizeof(FrameMol) * fm.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_fs, fs.data(), sizeof(double4) * fs.size(), hipMemcpyHostToDevice, e->stream));
     HIP_TRY(e, hipMemcpyAsync(d_ft, ft.data(), sizeof(int) * ft.size(), hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_fc, fcode.data(), sizeof(int) * fcode.size(), hipMemcpyHostToDevice, e->stream));
   }
   HIP_TRY(e, hipMemcpyAsync(d_tab, tab.data(), sizeof(double4) * tab.size(), hipMemcpyHostToDevice, e->stream));
-  HIP_TRY(e, hipMemcpyAsync(d_tab2, tab2.data(), sizeof(double4) * tab2.size(), hipMemcpyHostToDevice, e->stream));
+  if (!psite.empty())
+    HIP_TRY(e, hipMemcpyAsync(d_psite, psite.data(), sizeof(double4) * psite.size(), hipMemcpyHostToDevice, e->stream));
+  HIP_TRY(e, hipMemcpyAsync(d_img, imgtab.data(), sizeof(double4) * imgtab.size(), hipMemcpyHostToDevice, e->stream));
   if (!qtab.empty())
     HIP_TRY(e, hipMemcpyAsync(d_qtab, qtab.data(), sizeof(double4) * qtab.size(), hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipMemcpyAsync(d_q, w.q, sizeof(double) * kMaxWidomSites, hipMemcpyHostToDevice, e->stream));
@@ -1225,6 +1281,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     ra.coul_cbmc = coul_cbmc;
     ra.ntab = (int)qtab.size();
     ra.tab_inv_h = tab_inv_h;
+    ra.psite_stride = ps_stride;
     ra.rc_vdw2 = pp.vdw_style == CASS_VDW_NONE ? -1.0 : pp.rcut_vdwsq;
     ra.rc_coul2 = pp.charge_style == CASS_Q_NONE ? -1.0 : pp.rcut_coulsq;
     ra.rc_cbmc2 = pp.rcut_cbmcsq;
@@ -1253,8 +1310,8 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
                                d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
     HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
     const size_t shbytes = sizeof(double4) * std::max<size_t>(qtab.size(), 1);
-    hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs, d_fc,
-                       pos, live_sites, d_tab2, d_q, d_qtab, d_cnt, d_sp);
+    hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs,
+                       pos, live_sites, d_psite, d_img, d_qtab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
     if (w.charged && ew.nvecs > 0) {
