@@ -24,7 +24,7 @@ struct cass_box_state {
   int32_t nvecs = 0;
   std::vector<double> hx_h, hy_h, hz_h, cn_h;
   std::vector<int32_t> kn_h;        // (nx, ny, nz) per k, 3 ints
-  std::vector<int4> rows_h;         // rows of consecutive nx at fixed (ny, nz): (ny, nz, nx0, count)
+  std::vector<int4> rows_h;         // k traversal rows: (nx, nz, ny0, +-count)
   double *hx = nullptr, *hy = nullptr, *hz = nullptr, *cn = nullptr;
   double *cs = nullptr, *sn = nullptr;            // box structure factor
   double *cmol = nullptr, *smol = nullptr;        // [cap_mols][nvecs]

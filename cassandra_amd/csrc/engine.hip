@@ -137,6 +137,7 @@ static EwaldDev ewald_view(const cass_box_state &b, const int *row_k0) {
   w.sm_new = b.sm_new;
   w.rows = b.rows;
   w.row_k0 = row_k0;
+  w.kperm = row_k0 ? row_k0 + b.nrows : nullptr;
   return w;
 }
 
@@ -719,20 +720,48 @@ extern "C" int cass_ewald_setup(cass_engine *e, int32_t box, int32_t *nvecs) {
   }
   const int K = (int)hx.size();
   if (K > 100000) return cass_fail(e, CASS_ERR_KVEC, "Total number of k vectors exceeded");
-  // rows of consecutive nx at fixed (ny, nz) for the phase recurrence
+  // Traversal of the k set for the phase recurrences of the Widom kernels:
+  // at fixed (nz, nx), runs of consecutive ny that start next to ny = 0 and
+  // walk away from it (ny >= 0 upwards, ny < 0 downwards); rows ordered by
+  // nz, then nx.  kperm maps a traversal position to its k index.
   std::vector<int4> rows;
-  std::vector<int32_t> k0;
-  for (int k = 0; k < K; ++k) {
-    const int nx = kn[3 * k], ny = kn[3 * k + 1], nz = kn[3 * k + 2];
-    if (!rows.empty()) {
-      int4 &r = rows.back();
-      if (r.x == ny && r.y == nz && r.z + r.w == nx) {
-        r.w += 1;
-        continue;
+  std::vector<int32_t> k0, kperm;
+  {
+    std::vector<int> ord(K);
+    for (int k = 0; k < K; ++k) ord[k] = k;
+    std::sort(ord.begin(), ord.end(), [&](int a, int c) {
+      const int *na = &kn[3 * a], *nc = &kn[3 * c];
+      if (na[2] != nc[2]) return na[2] < nc[2];
+      if (na[0] != nc[0]) return na[0] < nc[0];
+      return na[1] < nc[1];
+    });
+    size_t g = 0;
+    while (g < ord.size()) {
+      size_t h = g;
+      const int nz = kn[3 * ord[g] + 2], nx = kn[3 * ord[g]];
+      while (h < ord.size() && kn[3 * ord[h] + 2] == nz && kn[3 * ord[h]] == nx) ++h;
+      // [g, h): one (nz, nx) set in increasing ny; split at ny = 0
+      size_t mid = g;
+      while (mid < h && kn[3 * ord[mid] + 1] < 0) ++mid;
+      for (int dir = 1; dir >= -1; dir -= 2) {
+        std::vector<int> part;
+        if (dir > 0)
+          for (size_t i = mid; i < h; ++i) part.push_back(ord[i]);
+        else
+          for (size_t i = mid; i > g; --i) part.push_back(ord[i - 1]);
+        for (size_t i = 0; i < part.size(); ++i) {
+          const int ny = kn[3 * part[i] + 1];
+          if (i > 0 && ny == kn[3 * part[i - 1] + 1] + dir) {
+            rows.back().w += dir;
+          } else {
+            rows.push_back(make_int4(nx, nz, ny, dir));
+            k0.push_back((int32_t)kperm.size());
+          }
+          kperm.push_back(part[i]);
+        }
       }
+      g = h;
     }
-    rows.push_back(make_int4(ny, nz, nx, 1));
-    k0.push_back(k);
   }
   // (re)allocate the k-space arrays of this box
   void *old[] = {b->hx, b->hy, b->hz, b->cn, b->cs, b->sn, b->cmol, b->smol, b->cs_new, b->sn_new,
@@ -752,14 +781,15 @@ extern "C" int cass_ewald_setup(cass_engine *e, int32_t box, int32_t *nvecs) {
   HIP_TRY(e, dalloc(&b->sm_new, K));
   HIP_TRY(e, dalloc(&b->cmol, (size_t)K * b->cap_mols));
   HIP_TRY(e, dalloc(&b->smol, (size_t)K * b->cap_mols));
-  // rows and their first-k table share one allocation
-  HIP_TRY(e, dalloc(&b->rows, rows.size() + (k0.size() + 3) / 4 + 1));
+  // rows, their first positions and the k permutation share one allocation
+  HIP_TRY(e, dalloc(&b->rows, rows.size() + (k0.size() + kperm.size() + 3) / 4 + 1));
   HIP_TRY(e, upload(e, b->hx, hx.data(), K));
   HIP_TRY(e, upload(e, b->hy, hy.data(), K));
   HIP_TRY(e, upload(e, b->hz, hz.data(), K));
   HIP_TRY(e, upload(e, b->cn, cn.data(), K));
   HIP_TRY(e, upload(e, b->rows, rows.data(), rows.size()));
   HIP_TRY(e, upload(e, (int32_t *)(b->rows + rows.size()), k0.data(), k0.size()));
+  HIP_TRY(e, upload(e, (int32_t *)(b->rows + rows.size()) + k0.size(), kperm.data(), kperm.size()));
   HIP_TRY(e, hipMemsetAsync(b->cs, 0, sizeof(double) * std::max(K, 1), e->stream));
   HIP_TRY(e, hipMemsetAsync(b->sn, 0, sizeof(double) * std::max(K, 1), e->stream));
   HIP_TRY(e, hipStreamSynchronize(e->stream));

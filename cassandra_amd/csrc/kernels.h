@@ -39,8 +39,9 @@ struct EwaldDev {
   const double *hx, *hy, *hz, *cn;
   double *cs, *sn, *cmol, *smol;
   double *cs_new, *sn_new, *cm_new, *sm_new;
-  const int4 *rows;      // (ny, nz, nx0, count), k index = running sum of counts
-  const int *row_k0;     // first k of each row
+  const int4 *rows;      // (nx, nz, ny0, +-count): a run of consecutive ny from ny0
+  const int *row_k0;     // first traversal position of each row
+  const int *kperm;      // k index of each traversal position
 };
 
 struct WidomArgs {

@@ -127,26 +127,31 @@ __device__ inline void place_probe_n(const WidomArgs &w, const BoxDev &b, uint64
 }
 
 // Per-k weights of the reciprocal change: (Cn, 2 Cn C_k, 2 Cn S_k, 0) from
-// the box structure factor (C_k, S_k) = sum_sites q (cos, sin)(h.r).
+// the box structure factor (C_k, S_k) = sum_sites q (cos, sin)(h.r), in the
+// order of the k traversal.
 __global__ void k_widom_weights(EwaldDev ew, double4 *__restrict__ wk) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= ew.nvecs) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ew.nvecs) return;
+  const int k = ew.kperm[j];
   const double cn = ew.cn[k];
-  wk[k] = make_double4(cn, 2.0 * cn * ew.cs[k], 2.0 * cn * ew.sn[k], 0.0);
+  wk[j] = make_double4(cn, 2.0 * cn * ew.cs[k], 2.0 * cn * ew.sn[k], 0.0);
 }
 
 // Reciprocal-space energy change of adding the test molecule:
 // charge_factor * sum_k Cn (2 C dC + dC^2 + 2 S dS + dS^2), written as
 // sum_k [Cn (dC^2 + dS^2) + (2 Cn C) dC + (2 Cn S) dS], with
-// q_a exp(i h.r_a) = q_a Ea^nx Eb^ny Ec^nz built by recurrences along the k
-// rows (h.r = 2 pi n.s, s the fractional coordinates, for the k sets of both
-// branches of Ewald_Reciprocal_Lattice_Vector_Setup): complex products across
-// rows, the two-term cosine recurrence within a row.  FP64 throughout.
+// q_a exp(i h.r_a) = q_a Ea^nx Eb^ny Ec^nz (h.r = 2 pi n.s, s the fractional
+// coordinates, for the k sets of both branches of
+// Ewald_Reciprocal_Lattice_Vector_Setup).  The k set is walked in runs of
+// consecutive ny at fixed (nz, nx), each starting next to ny = 0; the terms
+// follow the two-term cosine recurrence T(n + 1) = 2 cos(theta) T(n) - T(n - 1)
+// along nx (the run bases) and along ny (within a run).  FP64 throughout.
 template <int NS>
 __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__restrict__ wk, const BoxDev &b,
                                        const WidomArgs &w, const double (&px)[NS], const double (&py)[NS],
                                        const double (&pz)[NS]) {
-  cplx Ea[NS], Eb[NS], Ec[NS], Pb[NS], Pc[NS];
+  cplx Ea[NS], Eb[NS], Ec[NS], Pc[NS], B[NS], Bm[NS];
+  double C2a[NS], C2b[NS];
 #pragma unroll
   for (int a = 0; a < NS; ++a) {
     const double sx = b.Li[0] * px[a] + b.Li[3] * py[a] + b.Li[6] * pz[a];
@@ -159,44 +164,74 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__rest
     Eb[a] = {c, s};
     sincos(kTwoPi * sz, &s, &c);
     Ec[a] = {c, s};
-    Pb[a] = {1.0, 0.0};
-    Pc[a] = {1.0, 0.0};
+    C2a[a] = 2.0 * Ea[a].re;
+    C2b[a] = 2.0 * Eb[a].re;
+    Pc[a] = {w.q[a], 0.0};
+    B[a] = Pc[a];
+    Bm[a] = Pc[a];
   }
-  double C2[NS];
-#pragma unroll
-  for (int a = 0; a < NS; ++a) C2[a] = 2.0 * Ea[a].re;
-  int cur_ny = -1000000, cur_nz = -1000000;
+  int cur_nz = 0, cur_nx = -1000000;  // Pc holds q Ec^cur_nz
   double de = 0.0;
   for (int r = 0; r < ew.nrows; ++r) {
     const int4 row = ew.rows[r];
-    const int ny = row.x, nz = row.y, nx0 = row.z, cnt = row.w;
+    const int nx = row.x, nz = row.y, ny0 = row.z, cnt = row.w;
     if (nz != cur_nz) {
+      if (nz == cur_nz + 1) {
 #pragma unroll
-      for (int a = 0; a < NS; ++a) Pc[a] = cpow_int(Ec[a], nz);
+        for (int a = 0; a < NS; ++a) Pc[a] = cmul(Pc[a], Ec[a]);
+      } else {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+          const cplx p = cpow_int(Ec[a], nz);
+          Pc[a] = {w.q[a] * p.re, w.q[a] * p.im};
+        }
+      }
       cur_nz = nz;
-      cur_ny = -1000000;
+      cur_nx = -1000000;
     }
-    if (ny == cur_ny + 1) {
+    if (nx != cur_nx) {
+      // B: q Ea^nx Ec^nz, Bm: the same at nx - 1
+      if (nx == cur_nx + 1) {
 #pragma unroll
-      for (int a = 0; a < NS; ++a) Pb[a] = cmul(Pb[a], Eb[a]);
+        for (int a = 0; a < NS; ++a) {
+          const cplx bn = {fma(C2a[a], B[a].re, -Bm[a].re), fma(C2a[a], B[a].im, -Bm[a].im)};
+          Bm[a] = B[a];
+          B[a] = bn;
+        }
+      } else {
+#pragma unroll
+        for (int a = 0; a < NS; ++a) {
+          B[a] = cmul(Pc[a], cpow_int(Ea[a], nx));
+          Bm[a] = cmul(B[a], {Ea[a].re, -Ea[a].im});
+        }
+      }
+      cur_nx = nx;
+    }
+    // T: the site's term at ny0, U: at ny0 - dir
+    const int dir = cnt > 0 ? 1 : -1, len = cnt > 0 ? cnt : -cnt;
+    cplx T[NS], U[NS];
+    if (ny0 == 0) {
+#pragma unroll
+      for (int a = 0; a < NS; ++a) {
+        T[a] = B[a];
+        U[a] = cmul(B[a], {Eb[a].re, -dir * Eb[a].im});
+      }
+    } else if (ny0 == dir) {
+#pragma unroll
+      for (int a = 0; a < NS; ++a) {
+        T[a] = cmul(B[a], {Eb[a].re, dir * Eb[a].im});
+        U[a] = B[a];
+      }
     } else {
 #pragma unroll
-      for (int a = 0; a < NS; ++a) Pb[a] = cpow_int(Eb[a], ny);
-    }
-    cur_ny = ny;
-    // T: the site's term at nx, U: at nx - 1; along the row
-    // T(nx + 1) = 2 cos(theta) T(nx) - T(nx - 1), theta = 2 pi s_x
-    cplx T[NS], U[NS];
-#pragma unroll
-    for (int a = 0; a < NS; ++a) {
-      cplx base = cmul(Pb[a], Pc[a]);
-      if (nx0 != 0) base = cmul(base, cpow_int(Ea[a], nx0));
-      T[a] = {w.q[a] * base.re, w.q[a] * base.im};
-      U[a] = cmul(T[a], {Ea[a].re, -Ea[a].im});
+      for (int a = 0; a < NS; ++a) {
+        T[a] = cmul(B[a], cpow_int(Eb[a], ny0));
+        U[a] = cmul(T[a], {Eb[a].re, -dir * Eb[a].im});
+      }
     }
     const double4 *wr = wk + ew.row_k0[r];
 #pragma unroll 2
-    for (int kk = 0; kk < cnt; ++kk) {
+    for (int kk = 0; kk < len; ++kk) {
       double dc = T[0].re, ds = T[0].im;
 #pragma unroll
       for (int a = 1; a < NS; ++a) {
@@ -205,9 +240,9 @@ __device__ inline double recip_delta_n(const EwaldDev &ew, const double4 *__rest
       }
 #pragma unroll
       for (int a = 0; a < NS; ++a) {
-        const cplx nx = {fma(C2[a], T[a].re, -U[a].re), fma(C2[a], T[a].im, -U[a].im)};
+        const cplx tn = {fma(C2b[a], T[a].re, -U[a].re), fma(C2b[a], T[a].im, -U[a].im)};
         U[a] = T[a];
-        T[a] = nx;
+        T[a] = tn;
       }
       const double4 c = wr[kk];
       de = fma(c.x, fma(dc, dc, ds * ds), de);
