@@ -438,8 +438,12 @@ __device__ inline void site_pair_lj(double r2, double4 prm, double &e_v) {
     // v_rcp_f64 refined by one Newton step
     double y = __builtin_amdgcn_rcp(r2);
     y = fma(y, fma(-r2, y, 1.0), y);
-    const double s2 = prm.y * y, s6 = s2 * s2 * s2, s12 = s6 * s6;
-    e_v = prm.x * (s12 - s6) - prm.z;
+    const double s2 = prm.y * y, s6 = s2 * s2 * s2;
+    // 4 eps and the shift are scalar operands: a product and a difference
+    // (a fused form would need both in one instruction and a register copy)
+    double t = prm.x * fma(s6, s6, -s6);
+    asm("" : "+v"(t));
+    e_v = t - prm.z;
   } else {
     const float s2 = (float)prm.y * __builtin_amdgcn_rcpf((float)r2), s6 = s2 * s2 * s2, s12 = s6 * s6;
     e_v = (double)((float)prm.x * (s12 - s6) - (float)prm.z);
@@ -522,13 +526,14 @@ __device__ inline bool nonzero_bits(double v) {
 // minimum_image_separation.f90:35-60 for every pair inside the cutoff, and
 // any other pair stays outside the cutoff.  FP64 forms (p - s) + S, the
 // reference's own separation bit for bit; the other modes shift the lane's
-// site once per molecule, (p + S) - s.  coul_cbmc: the CBMC energy includes
-// the Coulomb term.  Lanes with live false (copies of a live site of
+// site once per molecule, (p + S) - s.  ecb and ecbq collect the LJ and
+// Coulomb terms inside the CBMC cutoff.  Lanes with live false (copies of a live site of
 // the same wave) compute but add nothing.
 template <typename P, int QS, bool SAME, bool UNI>
 __device__ inline void mol_pairs(const RealArgs &ra, const double4 *__restrict__ ps,
                                  const double4 *__restrict__ pp, int n, double4 sh, const double4 *qtab, bool live,
-                                 bool coul_cbmc, double px, double py, double pz, double &ev, double &eq, double &ecb) {
+                                 double px, double py, double pz, double &ev, double &eq, double &ecb,
+                                 double &ecbq) {
   constexpr bool kExactSep = std::is_same<P, PrecFP64>::value;
   double qx = px, qy = py, qz = pz;
   if (UNI && !kExactSep) {
@@ -573,7 +578,7 @@ __device__ inline void mol_pairs(const RealArgs &ra, const double4 *__restrict__
         double e_q;
         site_pair_qq<P, QS>(ra, r2, prm.w, prm.w, qtab, e_q);
         eq = fma(SAME ? wn : ((near && r2 <= ra.rc_coul2) ? 1.0 : 0.0), e_q, eq);
-        if (coul_cbmc) ecb = fma(wc, e_q, ecb);
+        ecbq = fma(wc, e_q, ecbq);
       }
     }
   }
@@ -611,10 +616,9 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
   const bool active = s < (int64_t)ra.ns * npad && idx < m;
   if (!wave_any(active)) return;
   const int v = order[(int64_t)a * m + (active ? idx : m - 1)];
-  const bool coul_cbmc = ra.coul_cbmc != 0;
   const double4 *psa = psite + (int64_t)a * ra.psite_stride;
   const double px = pos[(int64_t)v * 3], py = pos[(int64_t)v * 3 + 1], pz = pos[(int64_t)v * 3 + 2];
-  double ev = 0.0, eq = 0.0, ecb = 0.0;
+  double ev = 0.0, eq = 0.0, ecb = 0.0, ecbq = 0.0;
   // wave centre: the mean of the wave's sites, taken as minimum images
   // around the first lane's site (inactive lanes repeat a site of the wave)
   const double x0 = wave_uniform(px), y0 = wave_uniform(py), z0 = wave_uniform(pz);
@@ -664,14 +668,15 @@ __global__ void __launch_bounds__(kWBlock) k_widom_real(RealArgs ra, const Frame
       const int c = __builtin_amdgcn_readlane(code, l);
       const int first = c >> 13, nsite = (c >> 5) & 255;
       if (uni)
-        mol_pairs<P, QS, SAME, true>(ra, fsite + first, psa + first, nsite, imgtab[c & 31], qtab, active, coul_cbmc, px,
-                                     py, pz, ev, eq, ecb);
+        mol_pairs<P, QS, SAME, true>(ra, fsite + first, psa + first, nsite, imgtab[c & 31], qtab, active, px, py,
+                                     pz, ev, eq, ecb, ecbq);
       else
         mol_pairs<P, QS, SAME, false>(ra, fsite + first, psa + first, nsite, make_double4(0.0, 0.0, 0.0, 0.0),
-                                      qtab, active, coul_cbmc, px, py, pz, ev, eq, ecb);
+                                      qtab, active, px, py, pz, ev, eq, ecb, ecbq);
     }
   }
-  if (active) out[v] = make_double4(ev, eq, ecb, 0.0);
+  // the CBMC energy includes the Coulomb term of species with coul_cbmc
+  if (active) out[v] = make_double4(ev, eq, ra.coul_cbmc ? ecb + ecbq : ecb, 0.0);
 }
 
 // widom_var and the block sums of one insertion per lane (tail of the
