@@ -38,8 +38,28 @@ struct cass_box_state {
   double e_recip_staged = 0.0;
 };
 
+// Device tables of the frame for the Widom kernels (frame molecules and
+// sites, per-kind pair tables, image shifts, erfc table), kept between
+// batches and rebuilt when the engine generation, the box or the species
+// changes.
+struct WidomTables {
+  uint64_t gen = ~0ull;
+  int box = -1, species = -1;
+  char *dev = nullptr;
+  size_t cap = 0;
+  size_t o_fm = 0, o_fs = 0, o_ft = 0, o_tab = 0, o_psite = 0, o_img = 0, o_qtab = 0, o_q = 0;
+  int nmols = 0, ntab = 0;
+  bool split_ok = false;  // frame within the split kernels' packing limits
+  int64_t ps_stride = 0;
+  double tab_inv_h = 0.0, dcom_max = 0.0;
+};
+
 struct cass_engine {
   int device = 0;
+  // bumped by every call that changes topology, parameters, precision or a
+  // box's molecules (invalidates cached device tables)
+  uint64_t gen = 0;
+  WidomTables wtab;
   hipStream_t stream = nullptr;
   int32_t prec = CASS_PREC_FP64;
   std::string err;

@@ -237,7 +237,7 @@ extern "C" int cass_engine_destroy(cass_engine *e) {
   (void)hipStreamSynchronize(e->stream);
   for (auto &b : e->boxes) free_box(b);
   void *ps[] = {e->d_eps, e->d_sig, e->d_mn, e->d_mm, e->d_atom_type, e->d_atom_charge, e->d_atom_mass,
-                e->d_sp_natoms, e->d_sp_atom_off, e->d_sp_coul_cbmc, e->scratch, e->io};
+                e->d_sp_natoms, e->d_sp_atom_off, e->d_sp_coul_cbmc, e->scratch, e->io, e->wtab.dev};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (e->pinned) (void)hipHostFree(e->pinned);
@@ -270,12 +270,14 @@ extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const 
   }
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   for (auto &b : e->boxes) b.ewald_ready = b.ewald_ready && !atom_charge;
+  ++e->gen;
   return CASS_OK;
 }
 
 extern "C" int cass_set_precision(cass_engine *e, int32_t prec) {
   if (!e || (prec != CASS_PREC_FP64 && prec != CASS_PREC_MIXED && prec != CASS_PREC_FP32)) return CASS_ERR_ARG;
   e->prec = prec;
+  ++e->gen;
   return CASS_OK;
 }
 
@@ -301,6 +303,7 @@ extern "C" int cass_box_create(cass_engine *e, const cass_params *p, int32_t max
     return cass_fail(e, CASS_ERR_CUTOFF, "vdW cutoff exceeds half the box");
   HIP_TRY(e, hipSetDevice(e->device));
   e->boxes.emplace_back();
+  ++e->gen;
   cass_box_state &b = e->boxes.back();
   b.p = *p;
   b.cap_mols = max_mols;
@@ -353,6 +356,7 @@ extern "C" int cass_box_upload(cass_engine *e, int32_t box, int32_t nmols, const
   int rc = box_of(e, box, &b);
   if (rc) return rc;
   if (nmols < 0 || nmols > b->cap_mols) return cass_fail(e, CASS_ERR_ARG, "nmols exceeds box capacity");
+  ++e->gen;
   b->nmols = nmols;
   b->mol_species.assign(mol_species, mol_species + nmols);
   b->live.assign(live, live + nmols);
@@ -402,6 +406,7 @@ extern "C" int cass_mol_update(cass_engine *e, int32_t box, int32_t mol, int32_t
   if (rc) return rc;
   if (mol < 0 || mol >= b->nmols) return cass_fail(e, CASS_ERR_ARG, "bad molecule index");
   const int s = b->mol_species[mol], n = e->sp_natoms[s], f = b->mol_first[mol];
+  ++e->gen;
   // one pinned staging buffer: x[n], y[n], z[n], comd, live
   double *h = (double *)cass_pinned(e, sizeof(double) * (3 * n + 4) + 16);
   if (!h) return cass_fail(e, CASS_ERR_NOMEM, "pinned staging");

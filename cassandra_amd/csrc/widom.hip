@@ -1067,6 +1067,148 @@ static real_kernel_t real_kernel(int prec, int q_style, bool same) {
   return real_kernel_p<PrecFP64>(q_style, same);
 }
 
+// Builds the frame tables of a Widom batch on the host and uploads them to
+// e->wtab (kept until the engine generation, the box or the species
+// changes).  split: the batch takes the split kernels (the per-kind
+// frame-site records and the erfc table are only needed there).
+static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int species, const WidomArgs &w,
+                        bool split_style) {
+  const cass_params &pp = b.p;
+  const int ns = e->sp_natoms[species];
+  WidomTables &T = e->wtab;
+  // frame tables (host mirrors -> device)
+  std::vector<FrameMol> fm;
+  std::vector<double4> fs;
+  std::vector<int> ft;
+  int max_nat = 0;
+  fm.reserve(b.nmols);
+  for (int m = 0; m < b.nmols; ++m) {
+    if (!b.live[m]) continue;
+    const int sp = b.mol_species[m], nat = e->sp_natoms[sp], off = e->sp_atom_off[sp];
+    FrameMol r;
+    r.com = b.comd_h[m];
+    r.first = (int)fs.size();
+    r.n = nat;
+    r.pad0 = r.pad1 = 0;
+    fm.push_back(r);
+    const double *xyz = b.xyz_h.data() + 3 * (size_t)b.mol_first[m];
+    for (int a = 0; a < nat; ++a) {
+      fs.push_back(make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], e->atom_charge[off + a]));
+      ft.push_back(e->atom_type[off + a]);
+    }
+    max_nat = std::max(max_nat, nat);
+  }
+  const size_t nfs = fs.size();
+  // the real-space kernel reads 4 frame sites past a molecule's last and
+  // packs (first site, site count) into 19 + 8 bits
+  for (int k = 0; k < 4; ++k) fs.push_back(make_double4(0.0, 0.0, 0.0, 0.0));
+  const bool split_ok = nfs < ((size_t)1 << 19) && max_nat <= 255;
+  const bool split = split_style && split_ok;
+  const int nt = std::max(e->ntypes, 1);
+  std::vector<double4> tab((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
+  // split-path LJ table: (4 eps, sigma^2, shift, 0); the shift is the
+  // cut_shift constant 4 eps (c^12 - c^6), c^2 = sigma^2 / rc_vdw^2
+  std::vector<double4> tab2((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
+  for (int a = 0; a < ns; ++a)
+    for (int t = 0; t < e->ntypes; ++t) {
+      const int ti = w.type[a];
+      if (ti < 0 || pp.vdw_style == CASS_VDW_NONE) continue;
+      const size_t k = (size_t)ti * e->ntypes + t;
+      tab[(size_t)a * nt + t] = make_double4(e->eps[k], e->sig[k], e->mie_n[k], e->mie_m[k]);
+      const double eps = e->eps[k], sig = e->sig[k];
+      double shift = 0.0;
+      if (pp.vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+        const double c2 = sig * sig / pp.rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+        shift = 4.0 * eps * (c12 - c6);
+      }
+      tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
+    }
+
+  // per probe kind, one record per frame site: the LJ row entry and the
+  // charge product q_a q_j (times charge_factor in mixed precision, the
+  // factor the pair function of that mode leaves out)
+  const int64_t ps_stride = (int64_t)fs.size();
+  std::vector<double4> psite(split ? (size_t)ns * ps_stride : 0, make_double4(0.0, 0.0, 0.0, 0.0));
+  if (split)
+    for (int a = 0; a < ns; ++a)
+      for (size_t f = 0; f < nfs; ++f) {
+        const int tj = ft[f];
+        double4 r = tj >= 0 ? tab2[(size_t)a * nt + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
+        r.w = e->prec == CASS_PREC_MIXED ? (w.q[a] * kChargeFactor) * fs[f].w : w.q[a] * fs[f].w;
+        psite[(size_t)a * ps_stride + f] = r;
+      }
+  // image shifts (nx L, ny L, nz L) at (nx + 1) + 3 (ny + 1) + 9 (nz + 1)
+  std::vector<double4> imgtab(27);
+  for (int k = 0; k < 27; ++k)
+    imgtab[k] = make_double4((k % 3 - 1) * pp.basis_length[0], (k / 3 % 3 - 1) * pp.basis_length[1],
+                             (k / 9 - 1) * pp.basis_length[2], 0.0);
+
+  // Mixed precision, Ewald: cubic Hermite table of the A&S erfc(alpha r)
+  // over 0 <= r <= rc_pair (ntab intervals, rc_pair at u = ntab - 2)
+  std::vector<double4> qtab;
+  double tab_inv_h = 0.0;
+  if (split && pp.charge_style == CASS_Q_EWALD && e->prec == CASS_PREC_MIXED) {
+    const double xm = pp.alpha_ewald * w.rc_pair;
+    const int nt_q = (int)std::min(1024.0, std::max(64.0, std::ceil(xm / 0.004) + 2.0));
+    tab_inv_h = (nt_q - 2) / w.rc_pair;
+    const long double h = 1.0L / tab_inv_h, al = pp.alpha_ewald;
+    std::vector<long double> f(nt_q + 1), df(nt_q + 1);
+    for (int i = 0; i <= nt_q; ++i) erfc_as_host(al * h * i, al, f[i], df[i]);
+    qtab.resize(nt_q);
+    for (int i = 0; i < nt_q; ++i) {
+      const long double f0 = f[i], f1 = f[i + 1], d0 = h * df[i], d1 = h * df[i + 1];
+      qtab[i] = make_double4((double)f0, (double)d0, (double)(3 * (f1 - f0) - 2 * d0 - d1),
+                             (double)(2 * (f0 - f1) + d0 + d1));
+    }
+  }
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off = align256(off + std::max<size_t>(bytes, 1));
+    return o;
+  };
+  T.o_fm = carve(sizeof(FrameMol) * fm.size());
+  T.o_fs = carve(sizeof(double4) * fs.size());
+  T.o_ft = carve(sizeof(int) * ft.size());
+  T.o_tab = carve(sizeof(double4) * tab.size());
+  T.o_psite = carve(sizeof(double4) * psite.size());
+  T.o_img = carve(sizeof(double4) * imgtab.size());
+  T.o_qtab = carve(sizeof(double4) * qtab.size());
+  T.o_q = carve(sizeof(double) * kMaxWidomSites);
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  if (off > T.cap) {
+    if (T.dev) HIP_TRY(e, hipFree(T.dev));
+    T.dev = nullptr;
+    T.cap = 0;
+    HIP_TRY(e, hipMalloc((void **)&T.dev, off));
+    T.cap = off;
+  }
+  char *d = T.dev;
+  auto put = [&](size_t o, const void *src, size_t bytes) -> hipError_t {
+    return bytes ? hipMemcpyAsync(d + o, src, bytes, hipMemcpyHostToDevice, e->stream) : hipSuccess;
+  };
+  HIP_TRY(e, put(T.o_fm, fm.data(), sizeof(FrameMol) * fm.size()));
+  HIP_TRY(e, put(T.o_fs, fs.data(), sizeof(double4) * fs.size()));
+  HIP_TRY(e, put(T.o_ft, ft.data(), sizeof(int) * ft.size()));
+  HIP_TRY(e, put(T.o_tab, tab.data(), sizeof(double4) * tab.size()));
+  HIP_TRY(e, put(T.o_psite, psite.data(), sizeof(double4) * psite.size()));
+  HIP_TRY(e, put(T.o_img, imgtab.data(), sizeof(double4) * imgtab.size()));
+  HIP_TRY(e, put(T.o_qtab, qtab.data(), sizeof(double4) * qtab.size()));
+  HIP_TRY(e, put(T.o_q, w.q, sizeof(double) * kMaxWidomSites));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  T.nmols = (int)fm.size();
+  T.ntab = (int)qtab.size();
+  T.split_ok = split_ok;
+  T.ps_stride = ps_stride;
+  T.tab_inv_h = tab_inv_h;
+  T.dcom_max = 0.0;
+  for (const FrameMol &r : fm) T.dcom_max = std::max(T.dcom_max, r.com.w);
+  T.gen = e->gen;
+  T.box = box;
+  T.species = species;
+  return CASS_OK;
+}
+
 int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, uint64_t seed, const double *place,
                    double *widom_var, double *eout, cass_widom_result *res) {
   if (!e || box < 0 || box >= (int)e->boxes.size()) return CASS_ERR_ARG;
@@ -1108,90 +1250,17 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
                        pp.charge_style == CASS_Q_CUT) &&
                       bv_cubic(b);
 
-  // frame tables (host mirrors -> device)
-  std::vector<FrameMol> fm;
-  std::vector<double4> fs;
-  std::vector<int> ft;
-  int max_nat = 0;
-  fm.reserve(b.nmols);
-  for (int m = 0; m < b.nmols; ++m) {
-    if (!b.live[m]) continue;
-    const int sp = b.mol_species[m], nat = e->sp_natoms[sp], off = e->sp_atom_off[sp];
-    FrameMol r;
-    r.com = b.comd_h[m];
-    r.first = (int)fs.size();
-    r.n = nat;
-    r.pad0 = r.pad1 = 0;
-    fm.push_back(r);
-    const double *xyz = b.xyz_h.data() + 3 * (size_t)b.mol_first[m];
-    for (int a = 0; a < nat; ++a) {
-      fs.push_back(make_double4(xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2], e->atom_charge[off + a]));
-      ft.push_back(e->atom_type[off + a]);
+  // frame tables (host mirrors -> device), cached between batches
+  if (e->wtab.gen != e->gen || e->wtab.box != box || e->wtab.species != species) {
+    const int trc = widom_tables(e, b, box, species, w, common);
+    if (trc) {
+      e->wtab.gen = ~0ull;
+      return trc;
     }
-    max_nat = std::max(max_nat, nat);
   }
-  const size_t nfs = fs.size();
-  // the real-space kernel reads 4 frame sites past a molecule's last and
-  // packs (first site, site count) into 19 + 8 bits
-  for (int k = 0; k < 4; ++k) fs.push_back(make_double4(0.0, 0.0, 0.0, 0.0));
-  if (nfs >= ((size_t)1 << 19) || max_nat > 255) common = false;
+  const WidomTables &T = e->wtab;
+  common = common && T.split_ok;
   const int nt = std::max(e->ntypes, 1);
-  std::vector<double4> tab((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
-  // split-path LJ table: (4 eps, sigma^2, shift, 0); the shift is the
-  // cut_shift constant 4 eps (c^12 - c^6), c^2 = sigma^2 / rc_vdw^2
-  std::vector<double4> tab2((size_t)ns * nt, make_double4(0.0, 0.0, 0.0, 0.0));
-  for (int a = 0; a < ns; ++a)
-    for (int t = 0; t < e->ntypes; ++t) {
-      const int ti = w.type[a];
-      if (ti < 0 || pp.vdw_style == CASS_VDW_NONE) continue;
-      const size_t k = (size_t)ti * e->ntypes + t;
-      tab[(size_t)a * nt + t] = make_double4(e->eps[k], e->sig[k], e->mie_n[k], e->mie_m[k]);
-      const double eps = e->eps[k], sig = e->sig[k];
-      double shift = 0.0;
-      if (pp.vdw_sum_style == CASS_SUM_CUT_SHIFT) {
-        const double c2 = sig * sig / pp.rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
-        shift = 4.0 * eps * (c12 - c6);
-      }
-      tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
-    }
-
-  // per probe kind, one record per frame site: the LJ row entry and the
-  // charge product q_a q_j (times charge_factor in mixed precision, the
-  // factor the pair function of that mode leaves out)
-  const int64_t ps_stride = (int64_t)fs.size();
-  std::vector<double4> psite(common ? (size_t)ns * ps_stride : 0, make_double4(0.0, 0.0, 0.0, 0.0));
-  if (common)
-    for (int a = 0; a < ns; ++a)
-      for (size_t f = 0; f < nfs; ++f) {
-        const int tj = ft[f];
-        double4 r = tj >= 0 ? tab2[(size_t)a * nt + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
-        r.w = e->prec == CASS_PREC_MIXED ? (w.q[a] * kChargeFactor) * fs[f].w : w.q[a] * fs[f].w;
-        psite[(size_t)a * ps_stride + f] = r;
-      }
-  // image shifts (nx L, ny L, nz L) at (nx + 1) + 3 (ny + 1) + 9 (nz + 1)
-  std::vector<double4> imgtab(27);
-  for (int k = 0; k < 27; ++k)
-    imgtab[k] = make_double4((k % 3 - 1) * pp.basis_length[0], (k / 3 % 3 - 1) * pp.basis_length[1],
-                             (k / 9 - 1) * pp.basis_length[2], 0.0);
-
-  // Mixed precision, Ewald: cubic Hermite table of the A&S erfc(alpha r)
-  // over 0 <= r <= rc_pair (ntab intervals, rc_pair at u = ntab - 2)
-  std::vector<double4> qtab;
-  double tab_inv_h = 0.0;
-  if (common && pp.charge_style == CASS_Q_EWALD && e->prec == CASS_PREC_MIXED) {
-    const double xm = pp.alpha_ewald * w.rc_pair;
-    const int nt_q = (int)std::min(1024.0, std::max(64.0, std::ceil(xm / 0.004) + 2.0));
-    tab_inv_h = (nt_q - 2) / w.rc_pair;
-    const long double h = 1.0L / tab_inv_h, al = pp.alpha_ewald;
-    std::vector<long double> f(nt_q + 1), df(nt_q + 1);
-    for (int i = 0; i <= nt_q; ++i) erfc_as_host(al * h * i, al, f[i], df[i]);
-    qtab.resize(nt_q);
-    for (int i = 0; i < nt_q; ++i) {
-      const long double f0 = f[i], f1 = f[i + 1], d0 = h * df[i], d1 = h * df[i + 1];
-      qtab[i] = make_double4((double)f0, (double)d0, (double)(3 * (f1 - f0) - 2 * d0 - d1),
-                             (double)(2 * (f0 - f1) + d0 + d1));
-    }
-  }
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;
   const int nblk = (int)((n + kWBlock - 1) / kWBlock);
@@ -1224,14 +1293,6 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const size_t o_k1 = carve(sizeof(unsigned) * nslot);
   const size_t o_v0 = carve(sizeof(int) * nslot);
   const size_t o_v1 = carve(sizeof(int) * nslot);
-  const size_t o_fm = carve(sizeof(FrameMol) * fm.size());
-  const size_t o_fs = carve(sizeof(double4) * fs.size());
-  const size_t o_ft = carve(sizeof(int) * ft.size());
-  const size_t o_tab = carve(sizeof(double4) * tab.size());
-  const size_t o_psite = carve(sizeof(double4) * psite.size());
-  const size_t o_img = carve(sizeof(double4) * imgtab.size());
-  const size_t o_qtab = carve(sizeof(double4) * qtab.size());
-  const size_t o_q = carve(sizeof(double) * kMaxWidomSites);
   const size_t o_pos = carve(common && !place ? sizeof(double) * 3 * n * ns : 0);
   const size_t o_sp = carve(common ? sizeof(double4) * n * ns : 0);
   const size_t o_wk = carve(sizeof(double4) * std::max(b.nvecs, 1));
@@ -1247,32 +1308,19 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   int64_t *d_nov = (int64_t *)(d_sum + 1);
   unsigned *k0 = (unsigned *)(s + o_k0), *k1 = (unsigned *)(s + o_k1);
   int *v0 = (int *)(s + o_v0), *v1 = (int *)(s + o_v1);
-  FrameMol *d_fm = (FrameMol *)(s + o_fm);
-  double4 *d_fs = (double4 *)(s + o_fs);
-  int *d_ft = (int *)(s + o_ft);
-  double4 *d_tab = (double4 *)(s + o_tab);
-  double4 *d_psite = (double4 *)(s + o_psite);
-  double4 *d_img = (double4 *)(s + o_img);
-  double4 *d_qtab = (double4 *)(s + o_qtab);
-  double *d_q = (double *)(s + o_q);
+  FrameMol *d_fm = (FrameMol *)(T.dev + T.o_fm);
+  double4 *d_fs = (double4 *)(T.dev + T.o_fs);
+  int *d_ft = (int *)(T.dev + T.o_ft);
+  double4 *d_tab = (double4 *)(T.dev + T.o_tab);
+  double4 *d_psite = (double4 *)(T.dev + T.o_psite);
+  double4 *d_img = (double4 *)(T.dev + T.o_img);
+  double4 *d_qtab = (double4 *)(T.dev + T.o_qtab);
   double *d_pos = (double *)(s + o_pos);
   double4 *d_sp = (double4 *)(s + o_sp);
   double4 *d_wk = (double4 *)(s + o_wk);
   int *d_flag = (int *)(s + o_flag);
   double *d_recip = (double *)(s + o_recip);
   int *d_cnt = (int *)(s + o_cnt);
-  if (!fm.empty()) {
-    HIP_TRY(e, hipMemcpyAsync(d_fm, fm.data(), sizeof(FrameMol) * fm.size(), hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_fs, fs.data(), sizeof(double4) * fs.size(), hipMemcpyHostToDevice, e->stream));
-    HIP_TRY(e, hipMemcpyAsync(d_ft, ft.data(), sizeof(int) * ft.size(), hipMemcpyHostToDevice, e->stream));
-  }
-  HIP_TRY(e, hipMemcpyAsync(d_tab, tab.data(), sizeof(double4) * tab.size(), hipMemcpyHostToDevice, e->stream));
-  if (!psite.empty())
-    HIP_TRY(e, hipMemcpyAsync(d_psite, psite.data(), sizeof(double4) * psite.size(), hipMemcpyHostToDevice, e->stream));
-  HIP_TRY(e, hipMemcpyAsync(d_img, imgtab.data(), sizeof(double4) * imgtab.size(), hipMemcpyHostToDevice, e->stream));
-  if (!qtab.empty())
-    HIP_TRY(e, hipMemcpyAsync(d_qtab, qtab.data(), sizeof(double4) * qtab.size(), hipMemcpyHostToDevice, e->stream));
-  HIP_TRY(e, hipMemcpyAsync(d_q, w.q, sizeof(double) * kMaxWidomSites, hipMemcpyHostToDevice, e->stream));
 
   double *d_place = nullptr, *d_var = nullptr, *d_eout = nullptr;
   if (place) {
@@ -1291,7 +1339,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   PairConst<double> cd = make_pair_const<double>(b.p, e->ntypes);
   PairConst<float> cf = make_pair_const<float>(b.p, e->ntypes);
-  WidomFrame wf{d_fm, d_fs, d_ft, (int)fm.size()};
+  WidomFrame wf{d_fm, d_fs, d_ft, T.nmols};
   const int coul_cbmc = e->sp_coul_cbmc[species];
 
   HIP_TRY(e, hipEventRecord(e->evs[0], e->stream));
@@ -1316,12 +1364,12 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     ra.npad = (int64_t)((n + 63) / 64 * 64);
     ra.ns = ns;
     ra.ntypes = nt;
-    ra.nmols = (int)fm.size();
+    ra.nmols = T.nmols;
     ra.q_style = pp.charge_style;
     ra.coul_cbmc = coul_cbmc;
-    ra.ntab = (int)qtab.size();
-    ra.tab_inv_h = tab_inv_h;
-    ra.psite_stride = ps_stride;
+    ra.ntab = T.ntab;
+    ra.tab_inv_h = T.tab_inv_h;
+    ra.psite_stride = T.ps_stride;
     ra.rc_vdw2 = pp.vdw_style == CASS_VDW_NONE ? -1.0 : pp.rcut_vdwsq;
     ra.rc_coul2 = pp.charge_style == CASS_Q_NONE ? -1.0 : pp.rcut_coulsq;
     ra.rc_cbmc2 = pp.rcut_cbmcsq;
@@ -1329,7 +1377,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     ra.rc_pair2 = w.rc_pair2;
     ra.rc_pair = w.rc_pair;
     ra.alpha = pp.alpha_ewald;
-    for (const FrameMol &r : fm) ra.dcom_max = std::max(ra.dcom_max, r.com.w);
+    ra.dcom_max = T.dcom_max;
     ra.same_cut = (pp.vdw_style == CASS_VDW_NONE || pp.rcut_vdwsq == w.rc_pair2) &&
                   (pp.charge_style == CASS_Q_NONE || pp.rcut_coulsq == w.rc_pair2);
     for (int d = 0; d < 3; ++d) {
@@ -1349,7 +1397,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     HIP_TRY(e, rocprim::select((void *)(s + o_tmp), sort_tmp, rocprim::counting_iterator<int>(0), live_ins,
                                d_cnt + 1, n, LiveInsertion{d_flag}, e->stream));
     HIP_TRY(e, hipEventRecord(e->evs[1], e->stream));
-    const size_t shbytes = sizeof(double4) * std::max<size_t>(qtab.size(), 1);
+    const size_t shbytes = sizeof(double4) * std::max<size_t>(T.ntab, 1);
     hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs,
                        pos, live_sites, d_psite, d_img, d_qtab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
