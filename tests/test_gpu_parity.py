@@ -138,6 +138,43 @@ def test_widom_kernel_placements(case, probe):
     assert err3.max() < 1e-4
 
 
+def test_widom_tables_follow_state_changes():
+    """The frame tables that the Widom batch keeps on the device between calls
+    follow a molecule move and a change of precision: after each change the
+    batch equals that of a fresh engine built in the changed state."""
+    case, probe = WIDOM_CASES[2]
+    print(f"[gpu] widom tables after state changes {case}", flush=True)
+    fx, system, eng, box, _ = widom_box(case, probe, E.MIXED)
+    sp = len(system.species) - 1
+    place = fx["placements"]
+    _, _, eo0 = box.widom_placements(sp, place)
+    # move the frame molecule nearest the first surviving placement's centre
+    xyz = box.download()
+    nat = len(system.species[0].atoms)
+    first = int(np.flatnonzero(~fx["overlap"][:, 0])[0])
+    c = place[first].mean(axis=0)
+    mols = xyz[:3 * nat * system.nmols[0]].reshape(system.nmols[0], nat, 3)
+    d = mols.mean(axis=1) - c
+    d -= system.box * np.round(d / system.box)
+    mol = int(np.argmin((d * d).sum(axis=1)))
+    moved = mols[mol] + np.array([0.4, -0.3, 0.2])
+    box.mol_update(mol, moved.ravel())
+    _, _, eo1 = box.widom_placements(sp, place)
+    assert not np.array_equal(eo1[:, :3], eo0[:, :3])
+    fx2, system2, eng2, box2, _ = widom_box(case, probe, E.MIXED)
+    box2.mol_update(mol, moved.ravel())
+    _, _, eo2 = box2.widom_placements(sp, place)
+    assert np.array_equal(eo1, eo2)
+    # and a change of precision
+    eng.set_precision(E.FP64)
+    eng2.set_precision(E.FP64)
+    _, _, eo3 = box.widom_placements(sp, place)
+    _, _, eo4 = box2.widom_placements(sp, place)
+    assert np.array_equal(eo3, eo4)
+    assert not np.array_equal(eo3[:, :3], eo1[:, :3])
+    eng2.close()
+
+
 def test_ewald_move_commit_revert():
     """Update_System_Ewald_Reciprocal_Energy for a translation, then commit or
     revert: the staged total equals a full recompute at the new position and
