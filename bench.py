@@ -173,7 +173,7 @@ def cpu_baseline(system, sp, seconds=10.0, seed=11):
     return n / dt, n, dt
 
 
-def nvt_moves_per_s(nsteps=2000):
+def nvt_moves_per_s(nsteps=2000, device=False):
     from cassandra_amd import engine as E
     from cassandra_amd import systems as S
     system = S.config_spce512().parsed()
@@ -182,10 +182,10 @@ def nvt_moves_per_s(nsteps=2000):
     box.ewald_setup()
     e0 = box.compute_system_total_energy()
     r = E.rng(*system.seed)
-    box.nvt_run(r, 50, e0, record=False)
+    box.nvt_run(r, 50, e0, record=False, device=device)
     e0 = box.compute_system_total_energy()
     t0 = time.perf_counter()
-    box.nvt_run(r, nsteps, e0, record=False)
+    box.nvt_run(r, nsteps, e0, record=False, device=device)
     dt = time.perf_counter() - t0
     eng.close()
     return nsteps / dt
@@ -382,8 +382,10 @@ def main():
     if not a.quick:
         extra["plain_fp64_per_launch_trials_per_s"] = plain_fp64_per_launch(system, sp, 400, 3)
         log(f"plain fp64 per launch: {extra['plain_fp64_per_launch_trials_per_s']:.4g} trials/s")
-        extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(1000)
-        log(f"nvt: {extra['nvt_spce512_moves_per_s']:.4g} moves/s")
+        extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(20000, device=True)
+        extra["nvt_spce512_moves_per_s_host_loop"] = nvt_moves_per_s(1000)
+        log(f"nvt: device chain {extra['nvt_spce512_moves_per_s']:.4g} moves/s, "
+            f"host loop {extra['nvt_spce512_moves_per_s_host_loop']:.4g} moves/s")
         extra["widom_other_pairs"] = widom_configs(n, 3)
         log(f"widom pairs: { {k: round(v['trial_energies_per_s'] / 1e6, 1) for k, v in extra['widom_other_pairs'].items()} } M/s")
         extra["cbmc_gcmc_48_boxes"] = cbmc_gcmc()

@@ -80,6 +80,8 @@ def load_library(path=LIB_PATH):
         "cass_accept_or_reject": (i32, [P(abi.CassRng), d]),
         "cass_nvt_run": (i32, [vp, i32, P(abi.CassRng), P(abi.CassNvtOptions), i64, P(abi.CassEnergy),
                                P(abi.CassNvtRecord)]),
+        "cass_nvt_run_device": (i32, [vp, i32, P(abi.CassRng), P(abi.CassNvtOptions), i64, P(abi.CassEnergy),
+                               P(abi.CassNvtRecord)]),
     }
     sig.update({
         "cass_trial_energies_multi": (i32, [vp, i32, P(i32), P(i32), P(i32), P(i64), P(i64), P(d), i32, P(i32),
@@ -350,7 +352,10 @@ class Box:
         return r, wv, eo
 
     # ------------------------------------------------------------------- NVT
-    def nvt_run(self, rng: abi.CassRng, nsteps, energy: dict, species=0, record=True):
+    def nvt_run(self, rng: abi.CassRng, nsteps, energy: dict, species=0, record=True, device=False):
+        """nsteps NVT moves.  device=False: the host-driven loop (one set of
+        energy launches per move); device=True: the device-resident chain
+        (cass_nvt_run_device), the same draws and arithmetic in one workgroup."""
         sp = self.system.species[species]
         opt = abi.CassNvtOptions(self.params_prob_translate(), self.system.max_disp, self.system.max_rot_radians(),
                                  int(sp.linear), species)
@@ -361,8 +366,9 @@ class Box:
         lp = np.zeros(nsteps)
         rs = np.zeros(5 * nsteps, dtype=np.int64)
         rec = abi.CassNvtRecord(_p(kind, C.c_uint8), _p(acc, C.c_uint8), _p(et), _p(rs, C.c_int64), _p(lp))
-        self.eng._chk(self.lib.cass_nvt_run(self.eng.h, self.id, C.byref(rng), C.byref(opt), nsteps, C.byref(e),
-                                            C.byref(rec) if record else None), "cass_nvt_run")
+        fn = self.lib.cass_nvt_run_device if device else self.lib.cass_nvt_run
+        self.eng._chk(fn(self.eng.h, self.id, C.byref(rng), C.byref(opt), nsteps, C.byref(e),
+                         C.byref(rec) if record else None), "cass_nvt_run_device" if device else "cass_nvt_run")
         return dict(kind=kind, accepted=acc.astype(bool), energy=et, ln_pacc=lp, rng=rs.reshape(-1, 5),
                     final=e.as_dict())
 

@@ -364,6 +364,15 @@ typedef struct cass_nvt_record {
 int cass_nvt_run(cass_engine *eng, int32_t box, cass_rng *rng, const cass_nvt_options *opt,
                  int64_t nsteps, cass_energy *energy, cass_nvt_record *rec);
 
+/* The same move loop resident on the device: one workgroup runs the moves
+ * in sequence (draws, trial coordinates, old/new molecule energies, the
+ * staged reciprocal sums, accept_or_reject and the commit) with no host
+ * round trip per move; the host mirrors, *rng and *energy are brought up
+ * to date at the end.  Same arguments and results as cass_nvt_run
+ * (nvtmc_driver.f90:94-152); molecules of at most 64 sites. */
+int cass_nvt_run_device(cass_engine *eng, int32_t box, cass_rng *rng, const cass_nvt_options *opt,
+                        int64_t nsteps, cass_energy *energy, cass_nvt_record *rec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -198,24 +198,26 @@ def test_ewald_move_commit_revert():
     assert rel(box.ewald_full(), e_new2) < 1e-10
 
 
-def _nvt(case, nsteps):
+def _nvt(case, nsteps, device=False):
     fx = golden_npz(f"{case}_nvt.npz")
     system = config_by_name(case).parsed()
     eng, box = make_box(system)
     e0 = box.compute_system_total_energy()
     r = E.rng(*system.seed)
-    out = box.nvt_run(r, nsteps, e0)
+    out = box.nvt_run(r, nsteps, e0, device=device)
     return fx, system, box, out
 
 
+@pytest.mark.parametrize("device", [False, True], ids=["host_loop", "device_chain"])
 @pytest.mark.parametrize("case,nsteps", [("lj500", 100000), ("spce512", 100000)])
-def test_nvt_chain_replays_reference(case, nsteps):
+def test_nvt_chain_replays_reference(case, nsteps, device):
     """1e5 moves under the reference's seed: the same move kinds and the same
     accept/reject decisions as the reference; energies within 1e-10; a
     decision that differs only where u is within 1e-9 of exp(-ln_pacc) is
-    reported, not failed."""
-    print(f"[gpu] nvt chain {case} {nsteps}", flush=True)
-    fx, system, box, out = _nvt(case, nsteps)
+    reported, not failed.  Both the host-driven loop and the device-resident
+    chain."""
+    print(f"[gpu] nvt chain {case} {nsteps} device={device}", flush=True)
+    fx, system, box, out = _nvt(case, nsteps, device)
     kind_ref = np.unpackbits(fx["kind"])[:nsteps].astype(bool)
     acc_ref = np.unpackbits(fx["accept"])[:nsteps].astype(bool)
     assert np.array_equal(out["kind"].astype(bool), kind_ref)
