@@ -44,7 +44,90 @@ struct NvtChainArgs {
   double *energy, *ln_pacc;
   int64_t *rng_state;
   int *status;  // set to 1 when a molecule overlaps before its move
+  double *ctrav, *strav;  // molecule sums in k-traversal order (nvecs each)
+  // the box held in LDS for the whole chain (offsets in bytes)
+  int nsite, natoms_total, nspecies, ntypes;
+  int o_x, o_y, o_z, o_comd, o_first, o_spec, o_live, o_atype, o_acharge, o_spn, o_spo, o_pt, o_pq, o_prow,
+      o_mols, o_ent, o_eoff;
 };
+
+// One site pair of Compute_MoleculePair_Energy (energy_routines.f90:1589-1730):
+// site ia of the moving molecule (px/py/pz, type pt, charge pq) against site
+// (x, y, z, type tj, charge qj) of another molecule, full cutoffs, with the
+// reference's FP64 operations (the arithmetic of molpair_energy<PrecFP64>);
+// prow[ia * ntypes + tj] holds (eps, sigma, n, m) of the pair's types.
+// Returns true on overlap r^2 < rcut_low^2, else adds the pair's terms.
+// COMMON: LJ cut / cut_tail / cut_shift (or none) with Ewald, cut or no
+// Coulomb, whose cutoff tests and pair terms are pair_cutoff, lj_cut,
+// ewald_real and coul_cut directly (the branches vdw_inter and qq_inter take
+// for these styles).
+template <bool COMMON>
+__device__ inline bool nvt_pair(const BoxDev &b, const PairConst<double> &cd, double px, double py, double pz,
+                                int ti, double qi, const double4 *prow_i, double x, double y, double z, int tj,
+                                double qj, double &ev, double &eq) {
+  double dx = px - x, dy = py - y, dz = pz - z;
+  min_image(b, dx, dy, dz);
+  const double r2d = dx * dx + dy * dy + dz * dz;
+  if (r2d < cd.rc_low2) return true;
+  bool gv, gq;
+  if (COMMON) {
+    gv = cd.vdw_style == CASS_VDW_LJ && r2d <= cd.rc_vdw2;
+    gq = cd.q_style != CASS_Q_NONE && r2d <= cd.rc_coul2;
+  } else {
+    pair_cutoff(cd, false, r2d, gv, gq);
+  }
+  if (!(gv || gq)) return false;
+  double e_v = 0.0, e_q = 0.0;
+  if (ti >= 0 && tj >= 0 && gv) {
+    const double4 p = prow_i[tj];
+    e_v = COMMON ? lj_cut<double, MathRef>(cd, r2d, p.x, p.y) : vdw_inter<double, MathRef>(cd, r2d, p.x, p.y, p.z, p.w);
+  }
+  const bool any_q = cd.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
+  if (any_q && gq) {
+    if (COMMON)
+      e_q = cd.q_style == CASS_Q_EWALD ? ewald_real<double, MathRef>(cd, r2d, qi * qj)
+                                       : coul_cut<double, MathRef>(r2d, qi * qj);
+    else
+      e_q = qq_inter<double, MathRef>(cd, r2d, qi * qj);
+  }
+  if (gv) ev += e_v;
+  if (gq) eq += e_q;
+  return false;
+}
+
+// Exclusive workgroup scan of two counts (fixed order); also returns the
+// totals.  sc/st: LDS scratch of kNvtWaves ints each.
+__device__ inline void nvt_scan2(int c, int t, int &cpre, int &tpre, int &ctot, int &ttot, int *sc, int *st) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int ci = c, ti = t;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int cu = __shfl_up(ci, o, 64), tu = __shfl_up(ti, o, 64);
+    if (lane >= o) {
+      ci += cu;
+      ti += tu;
+    }
+  }
+  if (lane == 63) {
+    sc[wid] = ci;
+    st[wid] = ti;
+  }
+  __syncthreads();
+  int cw = 0, tw = 0, ca = 0, ta = 0;
+  for (int w = 0; w < kNvtWaves; ++w) {
+    if (w < wid) {
+      cw += sc[w];
+      tw += st[w];
+    }
+    ca += sc[w];
+    ta += st[w];
+  }
+  cpre = cw + ci - c;
+  tpre = tw + ti - t;
+  ctot = ca;
+  ttot = ta;
+  __syncthreads();
+}
 
 __device__ inline int64_t nvt_lsl(int64_t v, int k) { return (int64_t)((uint64_t)v << k); }
 __device__ inline int64_t nvt_lsr(int64_t v, int k) { return (int64_t)((uint64_t)v >> k); }
@@ -121,6 +204,7 @@ __device__ inline void nvt_fold(double &c, double *q, int n, double L, double h)
   }
 }
 
+template <bool COMMON, int NSE>
 __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
   __shared__ double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
   __shared__ double qx[kMaxSites], qy[kMaxSites], qz[kMaxSites];
@@ -129,17 +213,62 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
   __shared__ int s_m, s_rot, s_acc;
   __shared__ double sh[4][kNvtWaves];
   __shared__ int shf[kNvtWaves];
+  __shared__ int scan_c[kNvtWaves], scan_t[kNvtWaves];
+  __shared__ double eph[6 * kMaxSites];  // (cos, sin) of 2 pi s_x, s_y, s_z per new site
   __shared__ double res[4];
   __shared__ int resf;
+  extern __shared__ char lds[];
+  double *lx = (double *)(lds + a.o_x), *ly = (double *)(lds + a.o_y), *lz = (double *)(lds + a.o_z);
+  double4 *lcomd = (double4 *)(lds + a.o_comd);
+  int *lfirst = (int *)(lds + a.o_first), *lspec = (int *)(lds + a.o_spec), *llive = (int *)(lds + a.o_live);
+  int *atype = (int *)(lds + a.o_atype);
+  double *acharge = (double *)(lds + a.o_acharge);
+  int *spn = (int *)(lds + a.o_spn), *spo = (int *)(lds + a.o_spo);
+  int *pt = (int *)(lds + a.o_pt);
+  double *pq = (double *)(lds + a.o_pq);
+  double4 *prow = (double4 *)(lds + a.o_prow);
+  int *lmols = (int *)(lds + a.o_mols);
+  int *ent_j = (int *)(lds + a.o_ent), *ent_off = (int *)(lds + a.o_eoff);
   const int tid = threadIdx.x;
   const int n = a.n, K = a.ew.nvecs, nmols = a.b.nmols;
-  int64_t rs[5];
-  double en[5];
+  for (int i = tid; i < a.nsite; i += kNvtBlock) {
+    lx[i] = a.x[i];
+    ly[i] = a.y[i];
+    lz[i] = a.z[i];
+  }
+  for (int i = tid; i < nmols; i += kNvtBlock) {
+    lcomd[i] = a.comd[i];
+    lfirst[i] = a.b.first[i];
+    lspec[i] = a.b.species[i];
+    llive[i] = a.b.live[i];
+  }
+  for (int i = tid; i < a.natoms_total; i += kNvtBlock) {
+    atype[i] = a.tp.atom_type[i];
+    acharge[i] = a.tp.atom_charge[i];
+  }
+  for (int i = tid; i < a.nspecies; i += kNvtBlock) {
+    spn[i] = a.tp.sp_natoms[i];
+    spo[i] = a.tp.sp_atom_off[i];
+  }
+  for (int i = tid; i < a.nm; i += kNvtBlock) lmols[i] = a.mols[i];
+  for (int i = tid; i < n * a.ntypes; i += kNvtBlock) {
+    const int ia = i / a.ntypes, tj = i - ia * a.ntypes;
+    const int ti = a.tp.atom_type[a.spa0 + ia];
+    const int k = (ti < 0 ? 0 : ti) * a.ntypes + tj;
+    prow[i] = make_double4(a.tp.eps[k], a.tp.sig[k], a.tp.mn[k], a.tp.mm[k]);
+  }
+  if (tid < n) {
+    pspa[tid] = a.spa0 + tid;
+    pt[tid] = a.tp.atom_type[a.spa0 + tid];
+    pq[tid] = a.tp.atom_charge[a.spa0 + tid];
+  }
+  // RNG words and energies: used by thread 0 only, kept in LDS
+  __shared__ int64_t rs[5];
+  __shared__ double en[5];
   if (tid == 0) {
     for (int k = 0; k < 5; ++k) rs[k] = a.rng[k];
     for (int k = 0; k < 5; ++k) en[k] = a.en[k];
   }
-  if (tid < n) pspa[tid] = a.spa0 + tid;
   __syncthreads();
   for (int64_t st = 0; st < a.nsteps; ++st) {
     // --- the move (thread 0): draws in the order of nvtmc_driver / Translate / Rotate
@@ -148,14 +277,14 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
       const int rot = !(rno <= a.prob_translate);
       (void)nvt_uniform(rs);  // species pick (one mobile species)
       const int im = (int)(nvt_uniform(rs) * a.nm);
-      const int m = a.mols[im];
-      const int f = a.b.first[m];
-      const double4 co = a.comd[m];
+      const int m = lmols[im];
+      const int f = lfirst[m];
+      const double4 co = lcomd[m];
       double4 cn = co;
       for (int i = 0; i < n; ++i) {
-        ox[i] = a.x[f + i];
-        oy[i] = a.y[f + i];
-        oz[i] = a.z[f + i];
+        ox[i] = lx[f + i];
+        oy[i] = ly[f + i];
+        oz[i] = lz[f + i];
       }
       if (!rot) {
         const double dx = (2.0 * nvt_uniform(rs) - 1.0) * a.max_disp;
@@ -202,8 +331,13 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
             p2 = ang;
           else
             p3 = ang;
-          const double c1 = cos(p1), c2 = cos(p2), c3 = cos(p3);
-          const double s1 = sin(p1), s2 = sin(p2), s3 = sin(p3);
+          // cos(0) = 1 and sin(0) = 0 exactly: only the drawn axis needs them
+          const double ca = cos(ang), sa = sin(ang);
+          const double c1 = axis == 1 ? ca : 1.0, c2 = axis == 2 ? ca : 1.0, c3 = axis == 3 ? ca : 1.0;
+          const double s1 = axis == 1 ? sa : 0.0, s2 = axis == 2 ? sa : 0.0, s3 = axis == 3 ? sa : 0.0;
+          (void)p1;
+          (void)p2;
+          (void)p3;
           r[0] = c1 * c3 - s1 * c2 * s3;
           r[1] = s1 * c3 + c1 * c2 * s3;
           r[2] = s2 * s3;
@@ -231,18 +365,50 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     if (tid == 64) com_new = molecule_com(a.tp, a.spa0, n, qx, qy, qz);
     __syncthreads();
     const int m = s_m;
-    // --- inter energies of the old and the new placement (Compute_Molecule_Nonbond_Inter_Energy)
+    // --- inter energies of the old and the new placement (Compute_Molecule_Nonbond_Inter_Energy):
+    // the molecules that pass the COM prefilter (Check_MoleculePair_Cutoff)
+    // are listed in LDS, then every site pair of the list is one task
+    int nent = 0, ntask = 0;
+    for (int base = 0; base < 2 * nmols; base += kNvtBlock) {
+      const int t = base + tid;
+      int take = 0, tasks = 0, j = 0, which = 0;
+      if (t < 2 * nmols) {
+        which = t >= nmols ? 1 : 0;
+        j = t - which * nmols;
+        if (j != m && llive[j] && molpair_in_range(a.b, false, which ? com_new : com_old, lcomd[j])) {
+          take = 1;
+          tasks = n * spn[lspec[j]];
+        }
+      }
+      int cpre, tpre, ctot, ttot;
+      nvt_scan2(take, tasks, cpre, tpre, ctot, ttot, scan_c, scan_t);
+      if (take) {
+        ent_j[nent + cpre] = (j << 1) | which;
+        ent_off[nent + cpre] = ntask + tpre;
+      }
+      nent += ctot;
+      ntask += ttot;
+    }
+    if (tid == 0) ent_off[nent] = ntask;
+    __syncthreads();
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     int ov = 0;
-    for (int t = tid; t < 2 * nmols; t += kNvtBlock) {
-      const int which = t >= nmols ? 1 : 0;
-      const int j = t - which * nmols;
-      if (j == m || !a.b.live[j]) continue;
-      if (!molpair_in_range(a.b, false, which ? com_new : com_old, a.b.comd[j])) continue;
+    for (int task = tid; task < ntask; task += kNvtBlock) {
+      int lo = 0, hi = nent - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ent_off[mid] <= task)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const int code = ent_j[lo], which = code & 1, j = code >> 1;
+      const int sj = lspec[j], nj = spn[sj], oj = spo[sj], fj = lfirst[j];
+      const int local = task - ent_off[lo], ia = local / nj, ja = local - ia * nj;
+      const double *px = which ? qx : ox, *py = which ? qy : oy, *pz = which ? qz : oz;
       double ev = 0.0, eq = 0.0;
-      const bool o = molpair_energy<PrecFP64>(a.b, a.tp, a.cd, a.cf, false, a.coul_cbmc != 0, n, which ? qx : ox,
-                                              which ? qy : oy, which ? qz : oz, pspa, j, ev, eq, nullptr);
-      if (o) {
+      if (nvt_pair<COMMON>(a.b, a.cd, px[ia], py[ia], pz[ia], pt[ia], pq[ia], prow + ia * a.ntypes, lx[fj + ja], ly[fj + ja],
+                   lz[fj + ja], atype[oj + ja], acharge[oj + ja], ev, eq)) {
         ov |= 1 << which;
       } else {
         v[2 * which] += ev;
@@ -259,26 +425,98 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     double erm = 0.0;
     const bool reach = !(ov & 2);
     if (reach && a.charged) {
+      // unit phases exp(2 pi i s) of the new sites' fractional coordinates,
+      // one lane each; then each lane walks k runs of the traversal (ny
+      // runs at fixed (nz, nx)) with the cosine recurrence and stores the
+      // molecule sums at the run's k indices
+      if (NSE > 0 && tid < 3 * n) {
+        const int i = tid / 3, d = tid - 3 * i;
+        const double sd = a.b.Li[d] * qx[i] + a.b.Li[3 + d] * qy[i] + a.b.Li[6 + d] * qz[i];
+        double sv, cv;
+        sincos(kTwoPi * sd, &sv, &cv);
+        eph[2 * tid] = cv;
+        eph[2 * tid + 1] = sv;
+      }
+      __syncthreads();
       double e1[1] = {0.0};
       int f0 = 0;
-      for (int k = tid; k < K; k += kNvtBlock) {
-        const size_t mk = (size_t)m * K + k;
-        const double hx = a.ew.hx[k], hy = a.ew.hy[k], hz = a.ew.hz[k];
-        double c = 0.0, s = 0.0;
-        for (int i = 0; i < n; ++i) {
-          const double hr = hx * qx[i] + hy * qy[i] + hz * qz[i];
-          const double q = a.tp.atom_charge[a.spa0 + i];
-          double sv, cv;
-          sincos(hr, &sv, &cv);
-          c = c + q * cv;
-          s = s + q * sv;
+      double *__restrict__ ctrav = a.ctrav, *__restrict__ strav = a.strav;
+      const int *__restrict__ kperm = a.ew.kperm;
+      const double *__restrict__ ecn = a.ew.cn;
+      const double *__restrict__ ecs = a.ew.cs, *__restrict__ esn = a.ew.sn;
+      const double *__restrict__ ecm = a.ew.cmol, *__restrict__ esm = a.ew.smol;
+      double *__restrict__ cs_new = a.ew.cs_new, *__restrict__ sn_new = a.ew.sn_new;
+      double *__restrict__ cm_new = a.ew.cm_new, *__restrict__ sm_new = a.ew.sm_new;
+      if (NSE > 0) {
+        // molecule sums along the runs, in traversal order
+        const int4 *__restrict__ rows = a.ew.rows;
+        const int *__restrict__ row_k0 = a.ew.row_k0;
+        for (int r = tid; r < a.ew.nrows; r += kNvtBlock) {
+          const int4 row = rows[r];
+          const int nx = row.x, nz = row.y, ny0 = row.z, cnt = row.w;
+          const int dir = cnt > 0 ? 1 : -1, len = cnt > 0 ? cnt : -cnt;
+          const int j0 = row_k0[r];
+          cplx t[NSE > 0 ? NSE : 1], u[NSE > 0 ? NSE : 1];
+          double c2[NSE > 0 ? NSE : 1];
+#pragma unroll
+          for (int i = 0; i < NSE; ++i) {
+            const cplx ea = {eph[6 * i], eph[6 * i + 1]}, eb = {eph[6 * i + 2], eph[6 * i + 3]};
+            const cplx ec = {eph[6 * i + 4], eph[6 * i + 5]};
+            const cplx p = cmul(cmul(cpow_int(ea, nx), cpow_int(ec, nz)), cpow_int(eb, ny0));
+            t[i] = {pq[i] * p.re, pq[i] * p.im};
+            u[i] = cmul(t[i], {eb.re, -dir * eb.im});
+            c2[i] = 2.0 * eb.re;
+          }
+          for (int kk = 0; kk < len; ++kk) {
+            double c = t[0].re, sv = t[0].im;
+#pragma unroll
+            for (int i = 1; i < NSE; ++i) {
+              c = c + t[i].re;
+              sv = sv + t[i].im;
+            }
+#pragma unroll
+            for (int i = 0; i < NSE; ++i) {
+              const cplx tn = {fma(c2[i], t[i].re, -u[i].re), fma(c2[i], t[i].im, -u[i].im)};
+              u[i] = t[i];
+              t[i] = tn;
+            }
+            ctrav[j0 + kk] = c;
+            strav[j0 + kk] = sv;
+          }
         }
-        const double c1 = a.ew.cs[k] + (c - a.ew.cmol[mk]), s1 = a.ew.sn[k] + (s - a.ew.smol[mk]);
-        a.ew.cs_new[k] = c1;
-        a.ew.sn_new[k] = s1;
-        a.ew.cm_new[k] = c;
-        a.ew.sm_new[k] = s;
-        e1[0] += a.ew.cn[k] * (c1 * c1 + s1 * s1);
+        __threadfence_block();
+        __syncthreads();
+        // the staged box sums and the energy, one k per lane
+        for (int jt = tid; jt < K; jt += kNvtBlock) {
+          const int k = kperm[jt];
+          const size_t mk = (size_t)m * K + k;
+          const double c = ctrav[jt], sv = strav[jt];
+          const double c1 = ecs[k] + (c - ecm[mk]), s1 = esn[k] + (sv - esm[mk]);
+          cs_new[k] = c1;
+          sn_new[k] = s1;
+          cm_new[k] = c;
+          sm_new[k] = sv;
+          e1[0] += ecn[k] * (c1 * c1 + s1 * s1);
+        }
+      } else {
+        for (int k = tid; k < K; k += kNvtBlock) {
+          const size_t mk = (size_t)m * K + k;
+          const double hx = a.ew.hx[k], hy = a.ew.hy[k], hz = a.ew.hz[k];
+          double c = 0.0, sv = 0.0;
+          for (int i = 0; i < n; ++i) {
+            const double hr = hx * qx[i] + hy * qy[i] + hz * qz[i];
+            double sn, cs;
+            sincos(hr, &sn, &cs);
+            c = c + pq[i] * cs;
+            sv = sv + pq[i] * sn;
+          }
+          const double c1 = a.ew.cs[k] + (c - a.ew.cmol[mk]), s1 = a.ew.sn[k] + (sv - a.ew.smol[mk]);
+          a.ew.cs_new[k] = c1;
+          a.ew.sn_new[k] = s1;
+          a.ew.cm_new[k] = c;
+          a.ew.sm_new[k] = sv;
+          e1[0] += a.ew.cn[k] * (c1 * c1 + s1 * s1);
+        }
       }
       nvt_reduce<1>(e1, f0, sh, shf, res, &resf);
       erm = e1[0] * kChargeFactor;
@@ -309,7 +547,7 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
           if (a.charged) en[4] = erm;
           // the stored centre: the folded one after a translation, Get_COM
           // after a rotation (move_rotate.f90:287); max_dcom unchanged
-          double4 c = a.comd[m];
+          double4 c = lcomd[m];
           if (!s_rot) {
             c.x = cn_fold.x;
             c.y = cn_fold.y;
@@ -327,6 +565,7 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
             c.y = cy / tm;
             c.z = cz / tm;
           }
+          lcomd[m] = c;
           a.comd[m] = c;
         }
       }
@@ -341,8 +580,11 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     }
     __syncthreads();
     if (s_acc) {
-      const int f = a.b.first[m];
+      const int f = lfirst[m];
       if (tid < n) {
+        lx[f + tid] = qx[tid];
+        ly[f + tid] = qy[tid];
+        lz[f + tid] = qz[tid];
         a.x[f + tid] = qx[tid];
         a.y[f + tid] = qy[tid];
         a.z[f + tid] = qz[tid];
@@ -370,6 +612,8 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
 using namespace cass;
 
 BoxDev cass_box_view(const cass_engine *e, const cass_box_state &b);  // engine.hip
+extern "C" int cass_nvt_run(cass_engine *e, int32_t box, cass_rng *rng, const cass_nvt_options *opt,
+                            int64_t nsteps, cass_energy *en, cass_nvt_record *rec);  // nvt.cpp
 TopoDev cass_topo_view(const cass_engine *e);
 EwaldDev cass_ewald_view(const cass_box_state &b);
 
@@ -392,15 +636,65 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   if (nm == 0) return cass_fail(e, CASS_ERR_STATE, "no molecules to move");
   if (nsteps == 0) return CASS_OK;
   HIP_TRY(e, hipSetDevice(e->device));
+  // LDS layout of the box copy
+  const int nsite = (int)(b.xyz_h.size() / 3), nt = std::max(e->ntypes, 1);
+  size_t lds = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = lds;
+    lds = (lds + bytes + 15) & ~(size_t)15;
+    return (int)o;
+  };
+  NvtChainArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.o_x = take(sizeof(double) * nsite);
+  a.o_y = take(sizeof(double) * nsite);
+  a.o_z = take(sizeof(double) * nsite);
+  a.o_comd = take(sizeof(double4) * b.nmols);
+  a.o_first = take(sizeof(int) * b.nmols);
+  a.o_spec = take(sizeof(int) * b.nmols);
+  a.o_live = take(sizeof(int) * b.nmols);
+  a.o_atype = take(sizeof(int) * e->natoms_total);
+  a.o_acharge = take(sizeof(double) * e->natoms_total);
+  a.o_spn = take(sizeof(int) * e->nspecies);
+  a.o_spo = take(sizeof(int) * e->nspecies);
+  a.o_pt = take(sizeof(int) * n);
+  a.o_pq = take(sizeof(double) * n);
+  a.o_prow = take(sizeof(double4) * n * nt);
+  a.o_mols = take(sizeof(int) * nm);
+  a.o_ent = take(sizeof(int) * (2 * b.nmols + 1));
+  a.o_eoff = take(sizeof(int) * (2 * b.nmols + 1));
+  const bool common = (b.p.vdw_style == CASS_VDW_NONE ||
+                       (b.p.vdw_style == CASS_VDW_LJ &&
+                        (b.p.vdw_sum_style == CASS_SUM_CUT || b.p.vdw_sum_style == CASS_SUM_CUT_TAIL ||
+                         b.p.vdw_sum_style == CASS_SUM_CUT_SHIFT))) &&
+                      (b.p.charge_style == CASS_Q_NONE || b.p.charge_style == CASS_Q_EWALD ||
+                       b.p.charge_style == CASS_Q_CUT);
+  int lds_max = 0;
+  HIP_TRY(e, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, e->device));
+  // a box whose copy exceeds the workgroup's LDS runs the host-driven loop
+  if (lds > (size_t)lds_max - 8192) return cass_nvt_run(e, box, rng, opt, nsteps, en, rec);
+  // the reciprocal sums of molecules up to 4 sites by phase recurrences
+  // along the k runs, larger ones by sincos per k
+  const int nse = (charged && n <= 4 && b.nrows > 0) ? n : 0;
+  void (*const kt[2][5])(NvtChainArgs) = {
+      {k_nvt_chain<false, 0>, k_nvt_chain<false, 1>, k_nvt_chain<false, 2>, k_nvt_chain<false, 3>,
+       k_nvt_chain<false, 4>},
+      {k_nvt_chain<true, 0>, k_nvt_chain<true, 1>, k_nvt_chain<true, 2>, k_nvt_chain<true, 3>,
+       k_nvt_chain<true, 4>}};
+  void (*kern)(NvtChainArgs) = kt[common ? 1 : 0][nse];
+  HIP_TRY(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
 
   // device state of the chain: RNG words, energies, status, mobile list, records
   const size_t ns = (size_t)nsteps;
   int64_t *d_rng = nullptr;
-  HIP_TRY(e, hipMallocAsync((void **)&d_rng, sizeof(int64_t) * 5 + sizeof(double) * 5 + sizeof(int) * (2 + nm),
+  const size_t ktrav = charged ? (size_t)b.nvecs : 0;
+  HIP_TRY(e, hipMallocAsync((void **)&d_rng,
+                            sizeof(int64_t) * 5 + sizeof(double) * (5 + 2 * ktrav) + sizeof(int) * (2 + nm),
                             e->stream));
   double *d_en = (double *)(d_rng + 5);
-  int *d_status = (int *)(d_en + 5);
+  double *d_trav = d_en + 5;
+  int *d_status = (int *)(d_trav + 2 * ktrav);
   int *d_mols = d_status + 2;
   uint8_t *d_kind = nullptr, *d_acc = nullptr;
   double *d_energy = nullptr, *d_lnp = nullptr;
@@ -417,8 +711,6 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   HIP_TRY(e, hipMemcpyAsync(d_status, zero, sizeof(int) * 2, hipMemcpyHostToDevice, e->stream));
   HIP_TRY(e, hipMemcpyAsync(d_mols, mols.data(), sizeof(int) * nm, hipMemcpyHostToDevice, e->stream));
 
-  NvtChainArgs a;
-  std::memset(&a, 0, sizeof(a));
   a.b = cass_box_view(e, b);
   a.tp = cass_topo_view(e);
   a.cd = make_pair_const<double>(b.p, e->ntypes);
@@ -449,12 +741,18 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   a.ln_pacc = d_lnp;
   a.rng_state = d_rs;
   a.status = d_status;
+  a.ctrav = d_trav;
+  a.strav = d_trav + ktrav;
+  a.nsite = nsite;
+  a.natoms_total = e->natoms_total;
+  a.nspecies = e->nspecies;
+  a.ntypes = nt;
   // chunks of moves per launch (a launch runs a few tens of milliseconds)
   const int64_t chunk = 4096;
   for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
     a.step0 = s0;
     a.nsteps = std::min(chunk, nsteps - s0);
-    hipLaunchKernelGGL(k_nvt_chain, dim3(1), dim3(kNvtBlock), 0, e->stream, a);
+    hipLaunchKernelGGL(kern, dim3(1), dim3(kNvtBlock), lds, e->stream, a);
     HIP_TRY(e, hipGetLastError());
   }
   int64_t rs_out[5];
@@ -469,7 +767,6 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   if (d_lnp) HIP_TRY(e, hipMemcpyAsync(rec->ln_pacc, d_lnp, sizeof(double) * ns, hipMemcpyDeviceToHost, e->stream));
   if (d_rs) HIP_TRY(e, hipMemcpyAsync(rec->rng_state, d_rs, sizeof(int64_t) * 5 * ns, hipMemcpyDeviceToHost, e->stream));
   // host mirrors of the moved molecules
-  const size_t nsite = b.xyz_h.size() / 3;
   std::vector<double> hx(nsite), hy(nsite), hz(nsite);
   if (nsite) {
     HIP_TRY(e, hipMemcpyAsync(hx.data(), b.x, sizeof(double) * nsite, hipMemcpyDeviceToHost, e->stream));
@@ -482,7 +779,7 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
     if (p) HIP_TRY(e, hipFreeAsync(p, e->stream));
   HIP_TRY(e, hipFreeAsync(d_rng, e->stream));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
-  for (size_t i = 0; i < nsite; ++i) {
+  for (int i = 0; i < nsite; ++i) {
     b.xyz_h[3 * i] = hx[i];
     b.xyz_h[3 * i + 1] = hy[i];
     b.xyz_h[3 * i + 2] = hz[i];
