@@ -173,11 +173,11 @@ def cpu_baseline(system, sp, seconds=10.0, seed=11):
     return n / dt, n, dt
 
 
-def nvt_moves_per_s(nsteps=2000, device=False):
+def nvt_moves_per_s(nsteps=2000, device=False, mixed=False):
     from cassandra_amd import engine as E
     from cassandra_amd import systems as S
     system = S.config_spce512().parsed()
-    eng = E.Engine(system, precision=E.FP64)
+    eng = E.Engine(system, precision=E.MIXED if mixed else E.FP64)
     box = eng.add_box(system)
     box.ewald_setup()
     e0 = box.compute_system_total_energy()
@@ -383,8 +383,10 @@ def main():
         extra["plain_fp64_per_launch_trials_per_s"] = plain_fp64_per_launch(system, sp, 400, 3)
         log(f"plain fp64 per launch: {extra['plain_fp64_per_launch_trials_per_s']:.4g} trials/s")
         extra["nvt_spce512_moves_per_s"] = nvt_moves_per_s(20000, device=True)
+        extra["nvt_spce512_moves_per_s_mixed"] = nvt_moves_per_s(20000, device=True, mixed=True)
         extra["nvt_spce512_moves_per_s_host_loop"] = nvt_moves_per_s(1000)
-        log(f"nvt: device chain {extra['nvt_spce512_moves_per_s']:.4g} moves/s, "
+        log(f"nvt: device chain {extra['nvt_spce512_moves_per_s']:.4g} moves/s (FP64), "
+            f"{extra['nvt_spce512_moves_per_s_mixed']:.4g} (mixed), "
             f"host loop {extra['nvt_spce512_moves_per_s_host_loop']:.4g} moves/s")
         extra["widom_other_pairs"] = widom_configs(n, 3)
         log(f"widom pairs: { {k: round(v['trial_energies_per_s'] / 1e6, 1) for k, v in extra['widom_other_pairs'].items()} } M/s")

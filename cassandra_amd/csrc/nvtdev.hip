@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 #include <vector>
 
 #include "engine.h"
@@ -57,11 +58,13 @@ struct NvtChainArgs {
 // reference's FP64 operations (the arithmetic of molpair_energy<PrecFP64>);
 // prow[ia * ntypes + tj] holds (eps, sigma, n, m) of the pair's types.
 // Returns true on overlap r^2 < rcut_low^2, else adds the pair's terms.
-// COMMON: LJ cut / cut_tail / cut_shift (or none) with Ewald, cut or no
-// Coulomb, whose cutoff tests and pair terms are pair_cutoff, lj_cut,
-// ewald_real and coul_cut directly (the branches vdw_inter and qq_inter take
-// for these styles).
-template <bool COMMON>
+// PM: 0 any style with the reference's FP64 operations (vdw_inter,
+// qq_inter); 1 the common styles (LJ cut / cut_tail / cut_shift or none with
+// Ewald, cut or no Coulomb), whose cutoff tests and pair terms are
+// pair_cutoff, lj_cut, ewald_real and coul_cut directly (the branches
+// vdw_inter and qq_inter take for them), FP64 operations; 2 the common
+// styles in the mixed mode (hardware rcp/rsq seeds refined by Newton steps).
+template <int PM>
 __device__ inline bool nvt_pair(const BoxDev &b, const PairConst<double> &cd, double px, double py, double pz,
                                 int ti, double qi, const double4 *prow_i, double x, double y, double z, int tj,
                                 double qj, double &ev, double &eq) {
@@ -70,6 +73,8 @@ __device__ inline bool nvt_pair(const BoxDev &b, const PairConst<double> &cd, do
   const double r2d = dx * dx + dy * dy + dz * dz;
   if (r2d < cd.rc_low2) return true;
   bool gv, gq;
+  constexpr bool COMMON = PM > 0;
+  using M = typename std::conditional<PM == 2, MathFast, MathRef>::type;
   if (COMMON) {
     gv = cd.vdw_style == CASS_VDW_LJ && r2d <= cd.rc_vdw2;
     gq = cd.q_style != CASS_Q_NONE && r2d <= cd.rc_coul2;
@@ -80,13 +85,12 @@ __device__ inline bool nvt_pair(const BoxDev &b, const PairConst<double> &cd, do
   double e_v = 0.0, e_q = 0.0;
   if (ti >= 0 && tj >= 0 && gv) {
     const double4 p = prow_i[tj];
-    e_v = COMMON ? lj_cut<double, MathRef>(cd, r2d, p.x, p.y) : vdw_inter<double, MathRef>(cd, r2d, p.x, p.y, p.z, p.w);
+    e_v = COMMON ? lj_cut<double, M>(cd, r2d, p.x, p.y) : vdw_inter<double, MathRef>(cd, r2d, p.x, p.y, p.z, p.w);
   }
   const bool any_q = cd.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
   if (any_q && gq) {
     if (COMMON)
-      e_q = cd.q_style == CASS_Q_EWALD ? ewald_real<double, MathRef>(cd, r2d, qi * qj)
-                                       : coul_cut<double, MathRef>(r2d, qi * qj);
+      e_q = cd.q_style == CASS_Q_EWALD ? ewald_real<double, M>(cd, r2d, qi * qj) : coul_cut<double, M>(r2d, qi * qj);
     else
       e_q = qq_inter<double, MathRef>(cd, r2d, qi * qj);
   }
@@ -204,7 +208,7 @@ __device__ inline void nvt_fold(double &c, double *q, int n, double L, double h)
   }
 }
 
-template <bool COMMON, int NSE>
+template <int PM, int NSE>
 __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
   __shared__ double ox[kMaxSites], oy[kMaxSites], oz[kMaxSites];
   __shared__ double qx[kMaxSites], qy[kMaxSites], qz[kMaxSites];
@@ -391,7 +395,7 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     }
     if (tid == 0) ent_off[nent] = ntask;
     __syncthreads();
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double ev0 = 0.0, eq0 = 0.0, ev1 = 0.0, eq1 = 0.0;  // old / new placement
     int ov = 0;
     for (int task = tid; task < ntask; task += kNvtBlock) {
       int lo = 0, hi = nent - 1;
@@ -407,14 +411,18 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
       const int local = task - ent_off[lo], ia = local / nj, ja = local - ia * nj;
       const double *px = which ? qx : ox, *py = which ? qy : oy, *pz = which ? qz : oz;
       double ev = 0.0, eq = 0.0;
-      if (nvt_pair<COMMON>(a.b, a.cd, px[ia], py[ia], pz[ia], pt[ia], pq[ia], prow + ia * a.ntypes, lx[fj + ja], ly[fj + ja],
+      if (nvt_pair<PM>(a.b, a.cd, px[ia], py[ia], pz[ia], pt[ia], pq[ia], prow + ia * a.ntypes, lx[fj + ja], ly[fj + ja],
                    lz[fj + ja], atype[oj + ja], acharge[oj + ja], ev, eq)) {
-        ov |= 1 << which;
+        ov |= which ? 2 : 1;
+      } else if (which) {
+        ev1 += ev;
+        eq1 += eq;
       } else {
-        v[2 * which] += ev;
-        v[2 * which + 1] += eq;
+        ev0 += ev;
+        eq0 += eq;
       }
     }
+    double v[4] = {ev0, eq0, ev1, eq1};
     nvt_reduce<4>(v, ov, sh, shf, res, &resf);
     if (ov & 1) {
       if (tid == 0) *a.status = 1;
@@ -676,12 +684,14 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   // the reciprocal sums of molecules up to 4 sites by phase recurrences
   // along the k runs, larger ones by sincos per k
   const int nse = (charged && n <= 4 && b.nrows > 0) ? n : 0;
-  void (*const kt[2][5])(NvtChainArgs) = {
-      {k_nvt_chain<false, 0>, k_nvt_chain<false, 1>, k_nvt_chain<false, 2>, k_nvt_chain<false, 3>,
-       k_nvt_chain<false, 4>},
-      {k_nvt_chain<true, 0>, k_nvt_chain<true, 1>, k_nvt_chain<true, 2>, k_nvt_chain<true, 3>,
-       k_nvt_chain<true, 4>}};
-  void (*kern)(NvtChainArgs) = kt[common ? 1 : 0][nse];
+  void (*const kt[3][5])(NvtChainArgs) = {
+      {k_nvt_chain<0, 0>, k_nvt_chain<0, 1>, k_nvt_chain<0, 2>, k_nvt_chain<0, 3>, k_nvt_chain<0, 4>},
+      {k_nvt_chain<1, 0>, k_nvt_chain<1, 1>, k_nvt_chain<1, 2>, k_nvt_chain<1, 3>, k_nvt_chain<1, 4>},
+      {k_nvt_chain<2, 0>, k_nvt_chain<2, 1>, k_nvt_chain<2, 2>, k_nvt_chain<2, 3>, k_nvt_chain<2, 4>}};
+  // the mixed and FP32 modes take the mixed pair arithmetic for the common
+  // styles (FP64 accumulation, reciprocal space and acceptance in every mode)
+  const int pm = common ? (e->prec == CASS_PREC_FP64 ? 1 : 2) : 0;
+  void (*kern)(NvtChainArgs) = kt[pm][nse];
   HIP_TRY(e, hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   HIP_TRY(e, hipStreamSynchronize(e->stream));
 

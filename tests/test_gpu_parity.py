@@ -198,26 +198,29 @@ def test_ewald_move_commit_revert():
     assert rel(box.ewald_full(), e_new2) < 1e-10
 
 
-def _nvt(case, nsteps, device=False):
+def _nvt(case, nsteps, device=False, prec=None):
     fx = golden_npz(f"{case}_nvt.npz")
     system = config_by_name(case).parsed()
-    eng, box = make_box(system)
+    eng, box = make_box(system) if prec is None else make_box(system, prec)
     e0 = box.compute_system_total_energy()
     r = E.rng(*system.seed)
     out = box.nvt_run(r, nsteps, e0, device=device)
     return fx, system, box, out
 
 
-@pytest.mark.parametrize("device", [False, True], ids=["host_loop", "device_chain"])
+@pytest.mark.parametrize("device,prec", [(False, None), (True, None), (True, E.MIXED)],
+                         ids=["host_loop", "device_chain", "device_chain_mixed"])
 @pytest.mark.parametrize("case,nsteps", [("lj500", 100000), ("spce512", 100000)])
-def test_nvt_chain_replays_reference(case, nsteps, device):
+def test_nvt_chain_replays_reference(case, nsteps, device, prec):
     """1e5 moves under the reference's seed: the same move kinds and the same
     accept/reject decisions as the reference; energies within 1e-10; a
     decision that differs only where u is within 1e-9 of exp(-ln_pacc) is
-    reported, not failed.  Both the host-driven loop and the device-resident
-    chain."""
-    print(f"[gpu] nvt chain {case} {nsteps} device={device}", flush=True)
-    fx, system, box, out = _nvt(case, nsteps, device)
+    reported, not failed.  The host-driven loop and the device-resident chain
+    in FP64, and the device chain with the mixed-mode pair arithmetic (energies
+    within 1e-8 of the reference, the same decisions)."""
+    print(f"[gpu] nvt chain {case} {nsteps} device={device} prec={prec}", flush=True)
+    fx, system, box, out = _nvt(case, nsteps, device, prec)
+    tol = REL if prec is None else 1e-8
     kind_ref = np.unpackbits(fx["kind"])[:nsteps].astype(bool)
     acc_ref = np.unpackbits(fx["accept"])[:nsteps].astype(bool)
     assert np.array_equal(out["kind"].astype(bool), kind_ref)
@@ -228,6 +231,6 @@ def test_nvt_chain_replays_reference(case, nsteps, device):
         pytest.fail(f"first decision mismatch at move {i}: ln_pacc={lp!r}")
     ref = fx["energy_every100"][: nsteps // 100]
     got = out["energy"][99::100]
-    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) < REL
+    assert np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)) < tol
     full = box.compute_system_total_energy()["total"]
-    assert rel(out["final"]["total"], full) < REL
+    assert rel(out["final"]["total"], full) < tol
