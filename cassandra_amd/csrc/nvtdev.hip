@@ -37,6 +37,7 @@ struct NvtChainArgs {
   double4 *comd;      // the box's (COM, max_dcom) (the memory of b.comd)
   const int *mols;    // mobile molecules in locate order
   int nm, species, n, spa0, charged, linear, coul_cbmc;
+  int nj_uniform;  // the common site count of every molecule of the box, or 0
   double prob_translate, max_disp, max_rot, beta;
   int64_t nsteps, step0;  // moves of this launch; index of its first move
   int64_t *rng;           // 5 words, in/out
@@ -97,6 +98,19 @@ __device__ inline bool nvt_pair(const BoxDev &b, const PairConst<double> &cd, do
   if (gv) ev += e_v;
   if (gq) eq += e_q;
   return false;
+}
+
+// The centre prefilter of Check_MoleculePair_Cutoff (energy_routines.f90:
+// 3027-3088) on squared distances.  The reference compares the square root;
+// the two can differ only at the prefilter distance itself, where every site
+// pair lies beyond the cutoff and beyond rcut_low, so the energies and the
+// overlap test do not depend on the choice.
+__device__ inline bool nvt_in_range(const BoxDev &b, double4 ci, double4 cj) {
+  if (b.vdw_minimum) return true;
+  double dx = ci.x - cj.x, dy = ci.y - cj.y, dz = ci.z - cj.z;
+  min_image(b, dx, dy, dz);
+  const double rint = b.rcut_max + ci.w + cj.w;
+  return dx * dx + dy * dy + dz * dz <= rint * rint;
 }
 
 // Exclusive workgroup scan of two counts (fixed order); also returns the
@@ -179,16 +193,20 @@ __device__ inline void nvt_reduce(double (&v)[NV], int &flag, double (*sh)[kNvtW
     shf[wid] = flag;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (wid == 0) {
+    // the wave partials on the first kNvtWaves lanes, summed by a fixed
+    // shuffle tree
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      double s = 0.0;
-      for (int w = 0; w < kNvtWaves; ++w) s += sh[k][w];
-      res[k] = s;
+      double s = lane < kNvtWaves ? sh[k][lane] : 0.0;
+#pragma unroll
+      for (int o = kNvtWaves / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) res[k] = s;
     }
-    int f = 0;
-    for (int w = 0; w < kNvtWaves; ++w) f |= shf[w];
-    *resf = f;
+    int f = lane < kNvtWaves ? shf[lane] : 0;
+#pragma unroll
+    for (int o = kNvtWaves / 2; o > 0; o >>= 1) f |= __shfl_xor(f, o, 64);
+    if (lane == 0) *resf = f;
   }
   __syncthreads();
 #pragma unroll
@@ -379,7 +397,7 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
       if (t < 2 * nmols) {
         which = t >= nmols ? 1 : 0;
         j = t - which * nmols;
-        if (j != m && llive[j] && molpair_in_range(a.b, false, which ? com_new : com_old, lcomd[j])) {
+        if (j != m && llive[j] && nvt_in_range(a.b, which ? com_new : com_old, lcomd[j])) {
           take = 1;
           tasks = n * spn[lspec[j]];
         }
@@ -397,14 +415,21 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     __syncthreads();
     double ev0 = 0.0, eq0 = 0.0, ev1 = 0.0, eq1 = 0.0;  // old / new placement
     int ov = 0;
+    const int tpe = n * a.nj_uniform;  // tasks per listed molecule when all have nj_uniform sites
     for (int task = tid; task < ntask; task += kNvtBlock) {
-      int lo = 0, hi = nent - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (ent_off[mid] <= task)
-          lo = mid;
-        else
-          hi = mid - 1;
+      int lo;
+      if (tpe > 0) {
+        lo = task / tpe;
+      } else {
+        lo = 0;
+        int hi = nent - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (ent_off[mid] <= task)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
       }
       const int code = ent_j[lo], which = code & 1, j = code >> 1;
       const int sj = lspec[j], nj = spn[sj], oj = spo[sj], fj = lfirst[j];
@@ -754,6 +779,9 @@ extern "C" int cass_nvt_run_device(cass_engine *e, int32_t box, cass_rng *rng, c
   a.ctrav = d_trav;
   a.strav = d_trav + ktrav;
   a.nsite = nsite;
+  a.nj_uniform = b.nmols > 0 ? e->sp_natoms[b.mol_species[0]] : 0;
+  for (int mm = 0; mm < b.nmols; ++mm)
+    if (e->sp_natoms[b.mol_species[mm]] != a.nj_uniform) a.nj_uniform = 0;
   a.natoms_total = e->natoms_total;
   a.nspecies = e->nspecies;
   a.ntypes = nt;
