@@ -1,8 +1,8 @@
 !------------------------------------------------------------------------------
 ! cass_gpu_mod -- Fortran 2003 iso_c_binding interface to libcassgpu.so
 ! (include/cass_gpu.h).  A Cassandra build links this module and calls the
-! wrappers from the subroutines they replace; see INTEGRATION.md for the
-! call sites.  Indices are 1-based here (as in Cassandra) and converted to the
+! wrappers from the subroutines they replace (energy_routines.f90,
+! fragment_growth.f90, widom_insert.f90).  Indices are 1-based here (as in Cassandra) and converted to the
 ! C ABI's 0-based ones; LOGICAL results come back as default LOGICAL.
 !------------------------------------------------------------------------------
 MODULE cass_gpu_mod
