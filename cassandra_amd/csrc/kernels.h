@@ -158,22 +158,28 @@ __device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const 
       bool gv, gq;
       pair_cutoff(cd, cbmc, r2d, gv, gq);
       if (cbmc && !coul_cbmc) gq = false;
-      if (!(gv || gq) && !ecbmc) continue;
+      // the CBMC pass's own tests (Check_AtomPair_Cutoff with cbmc_flag)
+      bool gvc = false, gqc = false;
+      if (ecbmc) {
+        pair_cutoff(cd, true, r2d, gvc, gqc);
+        if (!coul_cbmc) gqc = false;
+      }
+      if (!(gv || gq || gvc || gqc)) continue;
       const int tj = tp.atom_type[oj + ja];
       TV e_v = (TV)0;
       TQ e_q = (TQ)0;
-      if (ti >= 0 && tj >= 0 && (gv || (ecbmc && r2d <= cd.rc_cbmc2))) {
+      if (ti >= 0 && tj >= 0 && (gv || gvc)) {
         const int k = ti * tp.ntypes + tj;
         e_v = vdw_inter<TV, typename P::M>(cv, (TV)r2d, (TV)tp.eps[k], (TV)tp.sig[k], (TV)tp.mn[k], (TV)tp.mm[k]);
       }
       const double qj = tp.atom_charge[oj + ja];
       const bool any_q = cd.q_style != CASS_Q_NONE && qi != 0.0 && qj != 0.0;
-      if (any_q && (gq || (ecbmc && r2d <= cd.rc_cbmc2))) e_q = qq_inter<TQ, typename P::M>(cq, (TQ)r2d, (TQ)(qi * qj));
+      if (any_q && (gq || gqc)) e_q = qq_inter<TQ, typename P::M>(cq, (TQ)r2d, (TQ)(qi * qj));
       if (gv) ev += (double)e_v;
       if (gq) eq += (double)e_q;
-      if (ecbmc && r2d <= cd.rc_cbmc2) {
-        bool cv = ti >= 0 && tj >= 0 && cd.vdw_style != CASS_VDW_NONE;
-        *ecbmc += (cv ? (double)e_v : 0.0) + ((any_q && coul_cbmc) ? (double)e_q : 0.0);
+      if (gvc || gqc) {
+        const bool cv = ti >= 0 && tj >= 0 && cd.vdw_style != CASS_VDW_NONE && gvc;
+        *ecbmc += (cv ? (double)e_v : 0.0) + ((any_q && gqc) ? (double)e_q : 0.0);
       }
     }
   }

@@ -308,9 +308,12 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
       if (!lanes_any(near)) continue;
       if (near) {
         if (r2 < cd.rc_low2) over = true;
-        bool gv, gq;
+        bool gv, gq, gvc, gqc;
         pair_cutoff(cd, false, r2, gv, gq);
-        const bool incb = r2 <= cd.rc_cbmc2;
+        // the CBMC pass's own tests (Check_AtomPair_Cutoff with cbmc_flag:
+        // the CBMC cutoff, except that minimum-image Coulomb keeps every pair)
+        pair_cutoff(cd, true, r2, gvc, gqc);
+        if (!coul_cbmc) gqc = false;
         const bool lj = tj >= 0 && w.type[a] >= 0 && cd.vdw_style != CASS_VDW_NONE;
         const double4 prm = lj ? pt.p[a * pt.ntypes + tj] : make_double4(0.0, 0.0, 0.0, 0.0);
         const bool do_v = lj && prm.x != 0.0;  // eps = 0 pairs add exactly 0
@@ -318,12 +321,12 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
         TV e_v = (TV)0;
         TQ e_q = (TQ)0;
         if (GEN) {
-          if (do_v && (gv || incb)) e_v = vdw_inter<TV, M>(cv, (TV)r2, (TV)prm.x, (TV)prm.y, (TV)prm.z, (TV)prm.w);
-          if (do_q && (gq || incb)) e_q = qq_inter<TQ, M>(cq, (TQ)r2, (TQ)(w.q[a] * st.w));
+          if (do_v && (gv || gvc)) e_v = vdw_inter<TV, M>(cv, (TV)r2, (TV)prm.x, (TV)prm.y, (TV)prm.z, (TV)prm.w);
+          if (do_q && (gq || gqc)) e_q = qq_inter<TQ, M>(cq, (TQ)r2, (TQ)(w.q[a] * st.w));
         } else {
           // LJ cut / cut_tail / cut_shift; Ewald real space or plain cut Coulomb
-          if (do_v && (gv || incb)) e_v = lj_cut<TV, M>(cv, (TV)r2, (TV)prm.x, (TV)prm.y);
-          if (do_q && (gq || incb)) {
+          if (do_v && (gv || gvc)) e_v = lj_cut<TV, M>(cv, (TV)r2, (TV)prm.x, (TV)prm.y);
+          if (do_q && (gq || gqc)) {
             if (cd.q_style == CASS_Q_EWALD)
               e_q = ewald_real<TQ, M>(cq, (TQ)r2, (TQ)(w.q[a] * st.w));
             else
@@ -332,7 +335,7 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
         }
         if (gv) ev += (double)e_v;
         if (gq) eq += (double)e_q;
-        if (incb) ecb += (double)e_v + (coul_cbmc ? (double)e_q : 0.0);
+        if (gvc || gqc) ecb += (gvc ? (double)e_v : 0.0) + (gqc ? (double)e_q : 0.0);
       }
     }
   }

@@ -138,6 +138,43 @@ def test_widom_kernel_placements(case, probe):
     assert err3.max() < 1e-4
 
 
+STYLE_CASES = ["sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut", "sty_shiftforce_dsfa", "sty_minimum_minimum",
+               "sty_charmm_ewald", "sty_ortho_ewald", "sty_mie_cut_tail", "sty_mie_cut_shift"]
+
+
+@pytest.mark.parametrize("case", STYLE_CASES)
+def test_widom_placements_every_style(case):
+    """cass_widom_placements for a CO2 probe in every pair-style case (the
+    fused per-insertion kernel for the styles outside the split pipeline, and
+    the orthorhombic box) against the oracle, which is pinned to the
+    reference's per-molecule energies of the same cases: FP64 within 1e-10,
+    identical overlap flags."""
+    from oracle import oracle as O
+    print(f"[gpu] widom every style {case}", flush=True)
+    system = S.with_probe(config_by_name(case), S.species_co2()).parsed()
+    eng, box = make_box(system, E.FP64)
+    e = box.compute_system_total_energy()
+    sp = len(system.species) - 1
+    probe = system.species[sp]
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rng = np.random.default_rng(17)
+    place = np.array([ref @ S.random_rotation(rng).T + (rng.random(3) - 0.5) * system.box for _ in range(300)])
+    _, _, eo = box.widom_placements(sp, place)
+    ob = O.OracleBox(system)
+    ob.total_energy()
+    out, ov = ob.widom_energies(sp, place)
+    assert np.array_equal(eo[:, 3].astype(bool), ov[:, 0])
+    ok = ~ov[:, 0]
+    assert ok.sum() > 20
+    assert np.max(np.abs(eo[ok, 0] - out[ok, 0]) / np.maximum(np.abs(out[ok, 0]), 1.0)) < REL
+    okc = ok & ~ov[:, 1]
+    assert np.max(np.abs(eo[okc, 1] - out[okc, 5]) / np.maximum(np.abs(out[okc, 5]), 1.0)) < REL
+    if system.charge_style == "ewald" and probe_charged(system):
+        de_ref = out[ok, 4] - e["reciprocal"]
+        assert np.max(np.abs(eo[ok, 2] - de_ref) / np.maximum(np.abs(out[ok, 4]), 1.0)) < REL
+    eng.close()
+
+
 def test_widom_tables_follow_state_changes():
     """The frame tables that the Widom batch keeps on the device between calls
     follow a molecule move and a change of precision: after each change the
