@@ -266,7 +266,14 @@ def main():
                     help="only the timed mixed-precision steps (for a profiler run of the measured kernel)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (scale for the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--nvt-only", action="store_true",
+                    help="only the device NVT chain on the 512-water box (for a profiler run)")
     a = ap.parse_args()
+    if a.nvt_only:
+        rate = nvt_moves_per_s(20000, device=True)
+        log(f"nvt device chain: {rate:.4g} moves/s")
+        print(json.dumps({"nvt_spce512_moves_per_s": rate}))
+        return
     if a.copy_calibration:
         import torch
         x = torch.ones(a.copy_calibration << 18, dtype=torch.float32, device="cuda")
