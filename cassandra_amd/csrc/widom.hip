@@ -1152,7 +1152,7 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
   double tab_inv_h = 0.0;
   if (split && pp.charge_style == CASS_Q_EWALD && e->prec == CASS_PREC_MIXED) {
     const double xm = pp.alpha_ewald * w.rc_pair;
-    const int nt_q = (int)std::min(1024.0, std::max(64.0, std::ceil(xm / 0.004) + 2.0));
+    const int nt_q = (int)std::min(1024.0, std::max(64.0, std::ceil(xm / 0.008) + 2.0));
     tab_inv_h = (nt_q - 2) / w.rc_pair;
     const long double h = 1.0L / tab_inv_h, al = pp.alpha_ewald;
     std::vector<long double> f(nt_q + 1), df(nt_q + 1);
