@@ -382,9 +382,19 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
       cn_fold = cn;
     }
     __syncthreads();
-    // the trial kernel's prefilter centres: Get_COM of each placement
+    // the trial kernel's prefilter centres: Get_COM of each placement; and,
+    // on a third wave, the unit phases exp(2 pi i s) of the new sites'
+    // fractional coordinates for the reciprocal sums
     if (tid == 0) com_old = molecule_com(a.tp, a.spa0, n, ox, oy, oz);
     if (tid == 64) com_new = molecule_com(a.tp, a.spa0, n, qx, qy, qz);
+    if (NSE > 0 && a.charged && tid >= 128 && tid < 128 + 3 * n) {
+      const int t = tid - 128, i = t / 3, d = t - 3 * i;
+      const double sd = a.b.Li[d] * qx[i] + a.b.Li[3 + d] * qy[i] + a.b.Li[6 + d] * qz[i];
+      double sv, cv;
+      sincos(kTwoPi * sd, &sv, &cv);
+      eph[2 * t] = cv;
+      eph[2 * t + 1] = sv;
+    }
     __syncthreads();
     const int m = s_m;
     // --- inter energies of the old and the new placement (Compute_Molecule_Nonbond_Inter_Energy):
@@ -458,19 +468,9 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
     double erm = 0.0;
     const bool reach = !(ov & 2);
     if (reach && a.charged) {
-      // unit phases exp(2 pi i s) of the new sites' fractional coordinates,
-      // one lane each; then each lane walks k runs of the traversal (ny
-      // runs at fixed (nz, nx)) with the cosine recurrence and stores the
-      // molecule sums at the run's k indices
-      if (NSE > 0 && tid < 3 * n) {
-        const int i = tid / 3, d = tid - 3 * i;
-        const double sd = a.b.Li[d] * qx[i] + a.b.Li[3 + d] * qy[i] + a.b.Li[6 + d] * qz[i];
-        double sv, cv;
-        sincos(kTwoPi * sd, &sv, &cv);
-        eph[2 * tid] = cv;
-        eph[2 * tid + 1] = sv;
-      }
-      __syncthreads();
+      // from the unit phases computed beside the centres, each lane walks
+      // k runs of the traversal (ny runs at fixed (nz, nx)) with the cosine
+      // recurrence and stores the molecule sums at the run's k indices
       double e1[1] = {0.0};
       int f0 = 0;
       double *__restrict__ ctrav = a.ctrav, *__restrict__ strav = a.strav;
@@ -586,17 +586,10 @@ __global__ void __launch_bounds__(kNvtBlock) k_nvt_chain(NvtChainArgs a) {
             c.y = cn_fold.y;
             c.z = cn_fold.z;
           } else {
-            double tm = 0.0, cx = 0.0, cy = 0.0, cz = 0.0;
-            for (int i = 0; i < n; ++i) {
-              const double ms = a.tp.atom_mass[a.spa0 + i];
-              tm = tm + ms;
-              cx = cx + ms * qx[i];
-              cy = cy + ms * qy[i];
-              cz = cz + ms * qz[i];
-            }
-            c.x = cx / tm;
-            c.y = cy / tm;
-            c.z = cz / tm;
+            // Get_COM of the rotated sites: the new placement's centre
+            c.x = com_new.x;
+            c.y = com_new.y;
+            c.z = com_new.z;
           }
           lcomd[m] = c;
           a.comd[m] = c;
