@@ -36,6 +36,12 @@ struct cass_box_state {
   bool ewald_ready = false;
   double e_recip = 0.0;             // last full / committed total reciprocal energy
   double e_recip_staged = 0.0;
+  // tabulated CBMC pair energies (cass_set_energy_table): apt_res bins of
+  // vdW per type pair, then the charge factor f2 per bin
+  double *apt = nullptr;
+  int32_t apt_res = 0;
+  double apt_rcsq_max = 0.0;        // the largest rcut_cbmc^2 over the boxes
+  double apt_step = 0.0, apt_shift = 0.0, apt_rcs = 0.0;
 };
 
 // Device tables of the frame for the Widom kernels (frame molecules and

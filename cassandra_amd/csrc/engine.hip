@@ -99,6 +99,11 @@ static BoxDev box_view(const cass_engine *e, const cass_box_state &b) {
   }
   v.rcut_max = b.p.rcut_max;
   v.rcut_cbmc = b.p.rcut_cbmc;
+  v.apt = b.apt;
+  v.apt_res = b.apt ? b.apt_res : 0;
+  v.apt_step = b.apt_step;
+  v.apt_shift = b.apt_shift;
+  v.apt_rcs = b.apt_rcs;
   (void)e;
   return v;
 }
@@ -226,7 +231,7 @@ extern "C" int cass_engine_create(int32_t device, const cass_topology *t, int32_
 
 static void free_box(cass_box_state &b) {
   void *ps[] = {b.x, b.y, b.z, b.site_spa, b.d_first, b.d_species, b.d_live, b.comd, b.hx, b.hy, b.hz,
-                b.cn, b.cs, b.sn, b.cmol, b.smol, b.cs_new, b.sn_new, b.cm_new, b.sm_new, b.rows};
+                b.cn, b.cs, b.sn, b.cmol, b.smol, b.cs_new, b.sn_new, b.cm_new, b.sm_new, b.rows, b.apt};
   for (void *p : ps)
     if (p) (void)hipFree(p);
 }
@@ -250,6 +255,111 @@ extern "C" int cass_engine_destroy(cass_engine *e) {
   return CASS_OK;
 }
 
+// AtomPair_VdW_Energy_Vector (energy_routines.f90:1733-1829) at one r^2 for
+// type pair k, in the reference's operation order.
+static double vdw_table_entry(const cass_engine *e, const cass_params &p, int k, double r2) {
+  const double eps = e->eps[k], sig = e->sig[k];
+  if (p.vdw_style == CASS_VDW_LJ) {
+    const double s2 = (sig * sig) / r2, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    double v = 4.0 * eps * (s12 - s6);
+    if (p.vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+      const double c2 = sig * sig / p.rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+      v = v - 4.0 * eps * (c12 - c6);
+    } else if (p.vdw_sum_style == CASS_SUM_CUT_SWITCH) {
+      if (r2 < p.ron_switch_sq) {
+      } else if (r2 > p.roff_switch_sq) {
+        v = 0.0;
+      } else {
+        const double d = p.roff_switch_sq - r2;
+        v = d * d * (p.switch_factor2 + 2.0 * r2) * p.switch_factor1 * v;
+      }
+    } else if (p.vdw_sum_style == CASS_SUM_CHARMM) {
+      v = eps * (s12 - 2.0 * s6);
+    } else if (p.vdw_sum_style == CASS_SUM_CUT_SHIFT_FORCE) {
+      const double c2 = sig * sig / p.rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+      const double rc = std::sqrt(p.rcut_vdwsq);
+      const double de = -(24.0 * eps * (2.0 * c12 - c6) / rc);
+      v = v - 4.0 * eps * (c12 - c6) - (std::sqrt(r2) - rc) * de;
+    }
+    return v;
+  }
+  if (p.vdw_style == CASS_VDW_MIE) {
+    const double n = e->mie_n[k], m = e->mie_m[k];
+    const double rc = std::sqrt(p.rcut_vdwsq);
+    const double coeff = n / (n - m) * std::pow(n / m, m / (n - m));
+    const double sr = sig / std::sqrt(r2);
+    double v = coeff * eps * (std::pow(sr, n) - std::pow(sr, m));
+    if (p.vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+      const double sc = sig / rc;
+      v = v - coeff * eps * (std::pow(sc, n) - std::pow(sc, m));
+    }
+    return v;
+  }
+  return 0.0;
+}
+
+// Create_Atompair_Nrg_table (atompair_nrg_table_routines.f90:149-261) for
+// one box: bins of width rsq_step = (max rcut_cbmc^2 - rcut_low^2) / res from
+// rsq_shifter = rcut_low^2 - rsq_step, each evaluated at its midpoint; the
+// vdW of every type pair and the charge factor f2 of the box's charge style
+// (libm erfc, which the Fortran ERFC intrinsic calls).  The q_i q_j factor is
+// applied at lookup.  Every species acts as both solute and solvent.
+static int build_energy_table(cass_engine *e, cass_box_state &b) {
+  const cass_params &p = b.p;
+  const int res = b.apt_res, nt = e->ntypes;
+  const double rsq_step = (b.apt_rcsq_max - p.rcut_lowsq) / res;
+  const double rsq_shifter = p.rcut_lowsq - rsq_step;
+  std::vector<double> mp(res), h((size_t)(nt * nt + 1) * res);
+  for (int i = 1; i <= res; ++i) {
+    const double lb = rsq_shifter + rsq_step * i;
+    mp[i - 1] = lb + 0.5 * rsq_step;
+  }
+  for (int k = 0; k < nt * nt; ++k)
+    for (int i = 0; i < res; ++i) h[(size_t)k * res + i] = vdw_table_entry(e, p, k, mp[i]);
+  double *f2 = h.data() + (size_t)nt * nt * res;
+  for (int i = 0; i < res; ++i) {
+    const double rij = std::sqrt(mp[i]);
+    if (p.charge_style == CASS_Q_EWALD) {
+      f2[i] = std::erfc(p.alpha_ewald * rij) / rij;
+    } else if (p.charge_style == CASS_Q_DSF) {
+      f2[i] = p.dsf_factor2 * (rij - p.rcut_coul) - p.dsf_factor1 + std::erfc(p.alpha_dsf * rij) / rij;
+    } else if (p.charge_style == CASS_Q_CUT) {
+      f2[i] = 1.0 / rij;
+    } else {
+      f2[i] = 0.0;
+    }
+  }
+  if (!b.apt) HIP_TRY(e, hipMalloc((void **)&b.apt, sizeof(double) * h.size()));
+  HIP_TRY(e, hipMemcpy(b.apt, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+  b.apt_step = rsq_step;
+  b.apt_shift = rsq_shifter;
+  b.apt_rcs = p.rcut_cbmcsq - rsq_shifter;
+  return CASS_OK;
+}
+
+extern "C" int cass_set_energy_table(cass_engine *e, int32_t box, int32_t res, double rcut_cbmcsq_max) {
+  if (!e || box < 0 || box >= (int32_t)e->boxes.size() || res < 0) return CASS_ERR_ARG;
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  cass_box_state &b = e->boxes[box];
+  if (res == 0) {
+    if (b.apt) HIP_TRY(e, hipFree(b.apt));
+    b.apt = nullptr;
+    b.apt_res = 0;
+    return CASS_OK;
+  }
+  const double rmax = rcut_cbmcsq_max > 0.0 ? rcut_cbmcsq_max : b.p.rcut_cbmcsq;
+  if (!(rmax >= b.p.rcut_cbmcsq) || !(rmax > b.p.rcut_lowsq))
+    return cass_fail(e, CASS_ERR_ARG, "energy table: need max rcut_cbmc^2 >= this box's and > rcut_low^2");
+  if (b.apt && b.apt_res != res) {
+    HIP_TRY(e, hipFree(b.apt));
+    b.apt = nullptr;
+  }
+  b.apt_res = res;
+  b.apt_rcsq_max = rmax;
+  return build_energy_table(e, b);
+}
+
 extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const double *sig,
                                        const double *atom_charge) {
   if (!e) return CASS_ERR_ARG;
@@ -269,7 +379,13 @@ extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const 
     HIP_TRY(e, upload(e, e->d_atom_charge, e->atom_charge.data(), e->natoms_total));
   }
   HIP_TRY(e, hipStreamSynchronize(e->stream));
-  for (auto &b : e->boxes) b.ewald_ready = b.ewald_ready && !atom_charge;
+  for (auto &b : e->boxes) {
+    b.ewald_ready = b.ewald_ready && !atom_charge;
+    if (b.apt) {
+      const int rc = build_energy_table(e, b);
+      if (rc != CASS_OK) return rc;
+    }
+  }
   ++e->gen;
   return CASS_OK;
 }

@@ -22,6 +22,12 @@ struct BoxDev {
   double L[3], hL[3];
   double Li[9], Lc[9];   // length_inv and length, column-major
   double rcut_max, rcut_cbmc;
+  // tabulated CBMC pair energies (Create_Atompair_Nrg_table): vdW per
+  // (type pair, bin) at apt[(ti * ntypes + tj) * apt_res + bin], the charge
+  // factor f2 per bin at apt[ntypes^2 * apt_res + bin]; apt_res = 0: off
+  const double *apt;
+  int apt_res;
+  double apt_step, apt_shift, apt_rcs;  // rsq_step, rsq_shifter, rcut_cbmcsq - rsq_shifter
 };
 
 // Topology view (nonbond tables and per species-atom data).
@@ -136,11 +142,52 @@ __host__ __device__ inline const PairConst<float> &pconst<float>(const PairConst
 // Cutoff and overlap tests use the FP64 r^2 in every mode, so all modes see
 // the same pair set.  With ecbmc the same pass also accumulates the
 // CBMC-cutoff sum.
+// Estimate_MoleculePair_Energy (energy_routines.f90:1528-1587): the CBMC
+// energy of the probe against molecule j from the pair table, added to ev
+// (the reference returns vdW and q-q together as the vdW term).  Each entry
+// is vdW(bin) + f2(bin) (q_i kChargeFactor) q_j, the sum
+// Create_Atompair_Nrg_table forms (atompair_nrg_table_routines.f90:244-256);
+// the bin is INT((r^2 - rsq_shifter) / rsq_step), 1-based.  Unfused
+// operations keep r^2 and the entry the reference's own roundings.  Returns
+// true on overlap (shifted r^2 below one step: r^2 < rcut_low^2).
+__device__ inline bool molpair_estimate(const BoxDev &b, const TopoDev &tp, int ni, const double *px,
+                                        const double *py, const double *pz, const int *pspa, int j,
+                                        double &ev) {
+  const int sj = b.species[j], fj = b.first[j];
+  const int nj = tp.sp_natoms[sj], oj = tp.sp_atom_off[sj];
+  const int nt = tp.ntypes, res = b.apt_res;
+  const double *f2 = b.apt + (size_t)nt * nt * res;
+  double mp = 0.0;
+  for (int ia = 0; ia < ni; ++ia) {
+    const int ti = tp.atom_type[pspa[ia]];
+    const double qf = __dmul_rn(tp.atom_charge[pspa[ia]], kChargeFactor);
+    for (int ja = 0; ja < nj; ++ja) {
+      double dx = px[ia] - b.x[fj + ja], dy = py[ia] - b.y[fj + ja], dz = pz[ia] - b.z[fj + ja];
+      min_image(b, dx, dy, dz);
+      const double r2s =
+          __dsub_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)), b.apt_shift);
+      if (!(r2s < b.apt_rcs)) continue;
+      if (r2s < b.apt_step) return true;
+      // 1 <= r2s / step < res + 1 here; the clamp guards the last edge
+      const int bin = max(0, min(res, (int)(r2s / b.apt_step)) - 1);
+      const int tj = tp.atom_type[oj + ja];
+      const double v = (ti >= 0 && tj >= 0) ? b.apt[(size_t)(tj * nt + ti) * res + bin] : 0.0;
+      const double e = __dadd_rn(v, __dmul_rn(f2[bin], __dmul_rn(qf, tp.atom_charge[oj + ja])));
+      mp = __dadd_rn(mp, e);
+    }
+  }
+  ev += mp;
+  return false;
+}
+
 template <typename P>
 __device__ inline bool molpair_energy(const BoxDev &b, const TopoDev &tp, const PairConst<double> &cd,
                                       const PairConst<float> &cf, bool cbmc, bool coul_cbmc, int ni,
                                       const double *px, const double *py, const double *pz,
                                       const int *pspa, int j, double &ev, double &eq, double *ecbmc) {
+  // CBMC trials with the pair table (Compute_Molecule_Nonbond_Inter_Energy,
+  // energy_routines.f90:1180-1182)
+  if (cbmc && b.apt_res > 0 && !ecbmc) return molpair_estimate(b, tp, ni, px, py, pz, pspa, j, ev);
   using TV = typename P::V;
   using TQ = typename P::Q;
   const PairConst<TV> &cv = pconst<TV>(cd, cf);

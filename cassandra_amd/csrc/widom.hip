@@ -1220,6 +1220,11 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const int ns = e->sp_natoms[species];
   if (ns > kMaxWidomSites || ns < 1) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
   if (n_ins * ns > (int64_t)INT32_MAX) return cass_fail(e, CASS_ERR_ARG, "too many insertions in one batch");
+  // the batch forms the CBMC column by direct pair terms; with the pair
+  // table the reference reads it from the table instead
+  // (energy_routines.f90:1275)
+  if (b.apt)
+    return cass_fail(e, CASS_ERR_ARG, "the Widom batch does not read the CBMC pair table; remove it first");
   WidomArgs w;
   int rc = cass_widom_prepare(e, b, species, w);
   if (rc) return rc;

@@ -62,6 +62,7 @@ def load_library(path=LIB_PATH):
         "cass_mol_inter_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(d), P(i32)]),
         "cass_mol_intra_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(i32)]),
         "cass_trial_energies": (i32, [vp, i32, i32, i32, P(d), i32, i32, P(d), P(i32)]),
+        "cass_set_energy_table": (i32, [vp, i32, i32, d]),
         "cass_total_energy": (i32, [vp, i32, i32, P(abi.CassEnergy), P(i32)]),
         "cass_lrc": (i32, [vp, i32, P(d), P(d)]),
         "cass_mol_self_energy": (i32, [vp, i32, i32, P(d)]),
@@ -264,6 +265,13 @@ class Box:
                                                      C.byref(b), C.byref(c), C.byref(ov)),
                       "cass_mol_intra_energy")
         return a.value, b.value, c.value, bool(ov.value)
+
+    def set_energy_table(self, res=1000, rcut_cbmcsq_max=0.0):
+        """The `energy_table` keyword: CBMC trials score molecule pairs from a
+        table of res bins (Create_Atompair_Nrg_table,
+        atompair_nrg_table_routines.f90:149-261); res=0 removes it."""
+        self.eng._chk(self.lib.cass_set_energy_table(self.eng.h, self.id, int(res), float(rcut_cbmcsq_max)),
+                      "cass_set_energy_table")
 
     def trial_energies(self, species, trial_xyz, cbmc=False, exclude=-1):
         t = np.ascontiguousarray(trial_xyz, dtype=np.float64)

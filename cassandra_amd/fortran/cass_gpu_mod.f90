@@ -151,6 +151,13 @@ MODULE cass_gpu_mod
        REAL(C_DOUBLE), INTENT(OUT) :: nrg(*)
        INTEGER(C_INT32_T), INTENT(OUT) :: overlap(*)
      END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_set_energy_table(eng, box, res, rcut_cbmcsq_max) &
+          BIND(C, NAME='cass_set_energy_table')
+       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box, res
+       REAL(C_DOUBLE), VALUE :: rcut_cbmcsq_max
+     END FUNCTION
      INTEGER(C_INT32_T) FUNCTION cass_widom_set_template(eng, box, species, xyz, e_intra_const) &
           BIND(C, NAME='cass_widom_set_template')
        IMPORT :: C_INT32_T, C_PTR, C_DOUBLE

@@ -205,6 +205,19 @@ int cass_trial_energies(cass_engine *eng, int32_t box, int32_t species, int32_t 
                         const double *trial_xyz, int32_t cbmc_flag, int32_t exclude_mol,
                         double *nrg, int32_t *overlap);
 
+/* Tabulated CBMC pair energies, the `energy_table` keyword: replaces
+ * Create_Atompair_Nrg_table (atompair_nrg_table_routines.f90:149-261) for
+ * one box, res bins (input_routines.f90:6043-6047; 1000 by default there)
+ * between rcut_low^2 and rcut_cbmcsq_max, the largest rcut_cbmc^2 over the
+ * system's boxes (<= 0: this box's own).  While set, every CBMC-cutoff
+ * molecule pair of the trial-set entry points (cass_trial_energies, _multi,
+ * plans, cass_mol_inter_energy with cbmc_flag) is scored by the table lookup
+ * of Estimate_MoleculePair_Energy (energy_routines.f90:1528-1587), as
+ * Compute_Molecule_Nonbond_Inter_Energy does (:1180-1182): vdW and q-q of
+ * the table together in e_vdw / nrg, the overlap test r^2 < rcut_low^2.
+ * res = 0 removes the table.  cass_set_nonbond_params rebuilds it. */
+int cass_set_energy_table(cass_engine *eng, int32_t box, int32_t res, double rcut_cbmcsq_max);
+
 /* Same, for the trial sets of several boxes in one persistent launch:
  * set s has ntrial[s] trials in box box_of_set[s], species species_of_set[s],
  * coordinates at trial_xyz + 3*xyz_offset[s], results at nrg/overlap +

@@ -42,6 +42,9 @@ typedef struct orc_box {
   double *cmol, *smol;          /* nmols*nvecs, molecule-major */
   double *cmol_save, *smol_save;
   int saved_mol;
+  /* tabulated CBMC energies (energy_table keyword): 0 bins = off */
+  int apt_res;
+  double apt_step, apt_shift, apt_rcs;
 } orc_box;
 
 /* -------------------------------------------------- params (derived block) */
@@ -351,6 +354,96 @@ static int molpair_energy(const orc_box *b, int cbmc, int si, const double *xi, 
   return 0;
 }
 
+/* AtomPair_VdW_Energy_Vector (energy_routines.f90:1733-1829) at one r^2:
+ * itype the solvent site's type, jtype the solute site's. */
+static double vdw_vector_entry(const orc_box *b, int itype, int jtype, double rijsq) {
+  const cass_params *p = &b->p;
+  const cass_topology *t = &b->t;
+  int k = itype * t->ntypes + jtype;
+  double eps = t->eps[k], sig = t->sig[k], e = 0.0;
+  if (p->vdw_style == CASS_VDW_LJ) {
+    double s2 = (sig * sig) / rijsq, s6 = s2 * s2 * s2, s12 = s6 * s6;
+    e = 4.0 * eps * (s12 - s6);
+    if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+      double c2 = sig * sig / p->rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+      e = e - 4.0 * eps * (c12 - c6);
+    } else if (p->vdw_sum_style == CASS_SUM_CUT_SWITCH) {
+      if (rijsq < p->ron_switch_sq) {
+      } else if (rijsq > p->roff_switch_sq) {
+        e = 0.0;
+      } else {
+        double d = p->roff_switch_sq - rijsq;
+        e = d * d * (p->switch_factor2 + 2.0 * rijsq) * p->switch_factor1 * e;
+      }
+    } else if (p->vdw_sum_style == CASS_SUM_CHARMM) {
+      e = eps * (s12 - 2.0 * s6);
+    } else if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT_FORCE) {
+      double c2 = sig * sig / p->rcut_vdwsq, c6 = c2 * c2 * c2, c12 = c6 * c6;
+      double rc = sqrt(p->rcut_vdwsq);
+      double dE = -(24.0 * eps * (2.0 * c12 - c6) / rc);
+      e = e - 4.0 * eps * (c12 - c6) - (sqrt(rijsq) - rc) * dE;
+    }
+  } else if (p->vdw_style == CASS_VDW_MIE) {
+    double n = t->mie_n[k], m = t->mie_m[k], rc = sqrt(p->rcut_vdwsq);
+    double coeff = n / (n - m) * pow(n / m, m / (n - m));
+    double sr = sig / sqrt(rijsq);
+    e = coeff * eps * (pow(sr, n) - pow(sr, m));
+    if (p->vdw_sum_style == CASS_SUM_CUT_SHIFT) {
+      double sc = sig / rc;
+      e = e - coeff * eps * (pow(sc, n) - pow(sc, m));
+    }
+  }
+  return e;
+}
+
+/* Entry `bin` (1-based) of Create_Atompair_Nrg_table
+ * (atompair_nrg_table_routines.f90:149-261) for a solute site (type ti,
+ * charge qi) and a solvent site (tj, qj): vdW + f2 * cfqq at the bin's
+ * midpoint, formed when it is read. */
+static double table_entry(const orc_box *b, int bin, int ti, double qi, int tj, double qj) {
+  const cass_params *p = &b->p;
+  double lb = b->apt_shift + b->apt_step * bin;
+  double mp = lb + 0.5 * b->apt_step;
+  double rij = sqrt(mp), f2 = 0.0;
+  if (p->charge_style == CASS_Q_EWALD) {
+    f2 = erfc(p->alpha_ewald * rij) / rij;
+  } else if (p->charge_style == CASS_Q_DSF) {
+    f2 = p->dsf_factor2 * (rij - p->rcut_coul) - p->dsf_factor1 + erfc(p->alpha_dsf * rij) / rij;
+  } else if (p->charge_style == CASS_Q_CUT) {
+    f2 = 1.0 / rij;
+  }
+  double v = (ti >= 0 && tj >= 0) ? vdw_vector_entry(b, tj, ti, mp) : 0.0;
+  double cfqq = (qi * CHARGE_FACTOR) * qj;
+  return v + f2 * cfqq;
+}
+
+/* Estimate_MoleculePair_Energy (energy_routines.f90:1528-1587): molecule i
+ * at (xi,yi,zi) of species si against stored molecule mj from the table. */
+static int molpair_estimate(const orc_box *b, int si, const double *xi, const double *yi,
+                            const double *zi, int mj, double *mpnrg) {
+  const cass_topology *t = &b->t;
+  int sj = b->mol_species[mj];
+  int oi = t->species_atom_offset[si], oj = t->species_atom_offset[sj];
+  int ni = t->species_natoms[si], nj = t->species_natoms[sj];
+  int fj = b->mol_first[mj];
+  *mpnrg = 0.0;
+  for (int ia = 0; ia < ni; ++ia) {
+    for (int ja = 0; ja < nj; ++ja) {
+      double dx, dy, dz;
+      min_image(&b->p, xi[ia] - b->x[fj + ja], yi[ia] - b->y[fj + ja], zi[ia] - b->z[fj + ja], &dx,
+                &dy, &dz);
+      double rs = dx * dx + dy * dy + dz * dz - b->apt_shift;
+      if (rs >= b->apt_rcs) continue;
+      if (rs < b->apt_step) return 1;
+      int bin = (int)(rs / b->apt_step);
+      if (bin > b->apt_res) bin = b->apt_res;  /* guards a rounding at the last edge */
+      *mpnrg = *mpnrg + table_entry(b, bin, t->atom_type[oi + ia], t->atom_charge[oi + ia],
+                                    t->atom_type[oj + ja], t->atom_charge[oj + ja]);
+    }
+  }
+  return 0;
+}
+
 /* Compute_Molecule_Nonbond_Inter_Energy (energy_routines.f90:1088-1206) for
  * a molecule of species si at (x,y,z) with COM com / max_dcom, skipping
  * stored molecule `self` (-1 none).  combine != 0 sums vdW + qq per molecule
@@ -365,8 +458,13 @@ static int mol_inter(const orc_box *b, int cbmc, int si, const double *x, const 
       if (b->mol_species[m] != s) continue;
       if (!b->mol_live[m] || m == self) continue;
       if (!molpair_cutoff(b, cbmc, com, dcom, &b->com[3 * m], b->max_dcom[m])) continue;
-      double v, q;
-      if (molpair_energy(b, cbmc, si, x, y, z, m, &v, &q)) return 1;
+      double v, q = 0.0;
+      if (cbmc && b->apt_res > 0) {
+        /* the table path of :1180-1182 and of the _Widom variant (:1275) */
+        if (molpair_estimate(b, si, x, y, z, m, &v)) return 1;
+      } else if (molpair_energy(b, cbmc, si, x, y, z, m, &v, &q)) {
+        return 1;
+      }
       if (combine) *evdw = *evdw + v + q;
       else {
         *eqq = *eqq + q;
@@ -715,6 +813,22 @@ int orc_mol_inter_energy(const orc_box *b, int m, int cbmc, double *evdw, double
 int orc_mol_intra_energy(const orc_box *b, int m, int cbmc, double *ev, double *eq, double *eqi) {
   int f = b->mol_first[m];
   return mol_intra(b, cbmc, b->mol_species[m], b->x + f, b->y + f, b->z + f, ev, eq, eqi);
+}
+
+/* energy_table: res bins between rcut_low^2 and rcsq_max (the largest
+ * rcut_cbmc^2 of the system; <= 0: this box's own); res = 0 turns it off
+ * (atompair_nrg_table_routines.f90:163-164). */
+int orc_set_energy_table(orc_box *b, int res, double rcsq_max) {
+  if (res <= 0) {
+    b->apt_res = 0;
+    return 0;
+  }
+  double rmax = rcsq_max > 0.0 ? rcsq_max : b->p.rcut_cbmcsq;
+  b->apt_res = res;
+  b->apt_step = (rmax - b->p.rcut_lowsq) / res;
+  b->apt_shift = b->p.rcut_lowsq - b->apt_step;
+  b->apt_rcs = b->p.rcut_cbmcsq - b->apt_shift;
+  return 0;
 }
 
 int orc_ewald_setup(orc_box *b) { return ewald_setup(b); }

@@ -48,6 +48,7 @@ def _lib():
     lib.orc_species_self.argtypes = [C.c_void_p, C.c_int]
     lib.orc_total_energy.argtypes = [C.c_void_p, P(abi.CassEnergy)]
     lib.orc_widom_energies.argtypes = [C.c_void_p, C.c_int, C.c_int, P(d), P(d), P(C.c_int)]
+    lib.orc_set_energy_table.argtypes = [C.c_void_p, C.c_int, d]
     lib.orc_rng_init.argtypes = [P(i64), i64, i64]
     lib.orc_rng_int.restype = i64
     lib.orc_rng_int.argtypes = [P(i64)]
@@ -135,6 +136,11 @@ class OracleBox:
         if rc:
             raise RuntimeError("overlap in configuration")
         return e.as_dict()
+
+    def set_energy_table(self, res=1000, rcsq_max=0.0):
+        """energy_table: CBMC-cutoff molecule pairs from the table of res bins
+        (res=0: off)."""
+        lib().orc_set_energy_table(self.h, int(res), float(rcsq_max))
 
     def widom_energies(self, species, placements):
         placements = np.ascontiguousarray(placements, dtype=np.float64)

@@ -142,6 +142,51 @@ STYLE_CASES = ["sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut", "sty_shiftfor
                "sty_charmm_ewald", "sty_ortho_ewald", "sty_mie_cut_tail", "sty_mie_cut_shift"]
 
 
+TABLE_CASES = ["spce512", "sty_shift_dsf", "sty_switch_cut", "sty_shiftforce_dsfa", "sty_charmm_ewald",
+               "sty_ortho_ewald", "sty_mie_cut_shift", "sty_minimum_minimum"]
+
+
+@pytest.mark.parametrize("case", TABLE_CASES)
+def test_trial_energies_energy_table(case):
+    """CBMC trial energies scored from the pair table (cass_set_energy_table,
+    the energy_table keyword) against the oracle's Estimate_MoleculePair_Energy
+    at a coarse and the default table: within 1e-12, identical overlap flags,
+    in every precision mode (the lookup is FP64 in all); removing the table
+    gives the direct CBMC energies back."""
+    from oracle import oracle as O
+    print(f"[gpu] energy table {case}", flush=True)
+    system = S.with_probe(config_by_name(case), S.species_co2()).parsed()
+    eng, box = make_box(system, E.FP64)
+    sp = len(system.species) - 1
+    probe = system.species[sp]
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rng = np.random.default_rng(29)
+    place = np.array([ref @ S.random_rotation(rng).T + (rng.random(3) - 0.5) * system.box for _ in range(256)])
+    ob = O.OracleBox(system)
+    direct, dov = ob.widom_energies(sp, place)
+    for res in (40, 1000):
+        ob.set_energy_table(res)
+        out, ov = ob.widom_energies(sp, place)
+        ob.set_energy_table(0)
+        box.set_energy_table(res)
+        for prec in (E.FP64, E.MIXED):
+            eng.set_precision(prec)
+            nrg, tov = box.trial_energies(sp, place, cbmc=True)
+            assert np.array_equal(tov, ov[:, 1])
+            ok = ~tov
+            assert ok.sum() > 20
+            assert np.max(np.abs(nrg[ok] - out[ok, 5]) / np.maximum(np.abs(out[ok, 5]), 1.0)) < 1e-12
+        eng.set_precision(E.FP64)
+        with pytest.raises(RuntimeError):
+            box.widom_placements(sp, place[:4])
+    box.set_energy_table(0)
+    nrg, tov = box.trial_energies(sp, place, cbmc=True)
+    assert np.array_equal(tov, dov[:, 1])
+    ok = ~tov
+    assert np.max(np.abs(nrg[ok] - direct[ok, 5]) / np.maximum(np.abs(direct[ok, 5]), 1.0)) < REL
+    eng.close()
+
+
 @pytest.mark.parametrize("case", STYLE_CASES)
 def test_widom_placements_every_style(case):
     """cass_widom_placements for a CO2 probe in every pair-style case (the
