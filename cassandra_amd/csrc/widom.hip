@@ -848,6 +848,58 @@ __global__ void __launch_bounds__(kWBlock) k_widom_finish(WidomArgs w, const int
   widom_finish(w, active, i, ev + eq, ecb, de_recip, over, part, part_ov, wvar, eout);
 }
 
+// The CBMC energy of every surviving insertion from the pair table, when one
+// is set (the table branch of Compute_Molecule_Nonbond_Inter_Energy_Widom,
+// energy_routines.f90:1275-1277): one lane per insertion over the frame
+// molecules in index order, with the CBMC-cutoff COM prefilter
+// (Check_MoleculePair_Cutoff).  The sum replaces the site partials' CBMC
+// term: site 0 carries it, the others 0, so the finish kernel adds it as it is.
+template <int NS>
+__global__ void __launch_bounds__(kWBlock) k_widom_cbmc_table(BoxDev b, TopoDev tp, int species,
+                                                              const double *__restrict__ pos,
+                                                              const int *__restrict__ live,
+                                                              const int *__restrict__ nlive, double4 *sp) {
+  const int64_t t = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
+  if (t >= *nlive) return;
+  const int64_t i = live[t];
+  const int oi = tp.sp_atom_off[species];
+  double px[NS], py[NS], pz[NS];
+  int pspa[NS];
+#pragma unroll
+  for (int a = 0; a < NS; ++a) {
+    px[a] = pos[(i * NS + a) * 3];
+    py[a] = pos[(i * NS + a) * 3 + 1];
+    pz[a] = pos[(i * NS + a) * 3 + 2];
+    pspa[a] = oi + a;
+  }
+  const double4 pc = molecule_com(tp, oi, NS, px, py, pz);
+  double e = 0.0;
+  for (int j = 0; j < b.nmols; ++j) {
+    if (!b.live[j] || !molpair_in_range(b, true, pc, b.comd[j])) continue;
+    // an overlap here is the r^2 < rcut_low^2 the overlap pass has flagged
+    if (molpair_estimate(b, tp, NS, px, py, pz, pspa, (int)j, e)) break;
+  }
+  sp[i * NS].z = e;
+#pragma unroll
+  for (int a = 1; a < NS; ++a) sp[i * NS + a].z = 0.0;
+}
+
+typedef void (*cbmc_table_kernel_t)(BoxDev, TopoDev, int, const double *, const int *, const int *, double4 *);
+
+static cbmc_table_kernel_t cbmc_table_kernel(int ns) {
+  switch (ns) {
+    case 1: return k_widom_cbmc_table<1>;
+    case 2: return k_widom_cbmc_table<2>;
+    case 3: return k_widom_cbmc_table<3>;
+    case 4: return k_widom_cbmc_table<4>;
+    case 5: return k_widom_cbmc_table<5>;
+    case 6: return k_widom_cbmc_table<6>;
+    case 7: return k_widom_cbmc_table<7>;
+    case 8: return k_widom_cbmc_table<8>;
+    default: return nullptr;
+  }
+}
+
 template <typename P, int NS, bool GEN>
 __global__ void __launch_bounds__(kWBlock) k_widom_sorted(WidomArgs w, BoxDev b, WidomFrame f, WidomPairTab pt,
                                                           EwaldDev ew, const double4 *__restrict__ wk,
@@ -995,6 +1047,7 @@ using namespace cass;
 
 BoxDev cass_box_view(const cass_engine *e, const cass_box_state &b);  // engine.hip
 EwaldDev cass_ewald_view(const cass_box_state &b);
+TopoDev cass_topo_view(const cass_engine *e);
 int cass_widom_prepare(cass_engine *e, cass_box_state &b, int species, WidomArgs &w);  // host.cpp
 
 static bool bv_cubic(const cass_box_state &b) { return b.p.box_shape == CASS_BOX_CUBIC; }
@@ -1220,11 +1273,6 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   const int ns = e->sp_natoms[species];
   if (ns > kMaxWidomSites || ns < 1) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
   if (n_ins * ns > (int64_t)INT32_MAX) return cass_fail(e, CASS_ERR_ARG, "too many insertions in one batch");
-  // the batch forms the CBMC column by direct pair terms; with the pair
-  // table the reference reads it from the table instead
-  // (energy_routines.f90:1275)
-  if (b.apt)
-    return cass_fail(e, CASS_ERR_ARG, "the Widom batch does not read the CBMC pair table; remove it first");
   WidomArgs w;
   int rc = cass_widom_prepare(e, b, species, w);
   if (rc) return rc;
@@ -1268,6 +1316,11 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   const WidomTables &T = e->wtab;
   common = common && T.split_ok;
+  // with the pair table the CBMC column is read from it
+  // (energy_routines.f90:1275); the split pipeline does that, the fused
+  // kernel of the other styles does not
+  if (b.apt && !common)
+    return cass_fail(e, CASS_ERR_ARG, "the pair table needs the split Widom pipeline (common pair styles, cubic box)");
   const int nt = std::max(e->ntypes, 1);
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;
@@ -1409,6 +1462,13 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     hipLaunchKernelGGL(real_kernel(e->prec, pp.charge_style, ra.same_cut != 0), dim3(rblk), dim3(kWBlock), shbytes, e->stream, ra, d_fm, d_fs,
                        pos, live_sites, d_psite, d_img, d_qtab, d_cnt, d_sp);
     HIP_TRY(e, hipGetLastError());
+    if (b.apt) {
+      cbmc_table_kernel_t ck = cbmc_table_kernel(ns);
+      if (!ck) return cass_fail(e, CASS_ERR_ARG, "test molecule too large");
+      hipLaunchKernelGGL(ck, dim3(nblk), dim3(kWBlock), 0, e->stream, bv, cass_topo_view(e), (int)species, pos,
+                         live_ins, d_cnt + 1, d_sp);
+      HIP_TRY(e, hipGetLastError());
+    }
     HIP_TRY(e, hipEventRecord(e->evs[2], e->stream));
     if (w.charged && ew.nvecs > 0) {
       hipLaunchKernelGGL(rk, dim3(nblk), dim3(kWBlock), 0, e->stream, w, bv, ew, d_wk, pos, live_ins, d_cnt + 1,
