@@ -42,6 +42,9 @@ struct cass_box_state {
   int32_t apt_res = 0;
   double apt_rcsq_max = 0.0;        // the largest rcut_cbmc^2 over the boxes
   double apt_step = 0.0, apt_shift = 0.0, apt_rcs = 0.0;
+  // per type-pair core radii^2 of the Widom overlap pre-screen
+  // (cass_widom_set_rminsq); empty: off
+  std::vector<double> rminsq;
 };
 
 // Device tables of the frame for the Widom kernels (frame molecules and
@@ -53,7 +56,9 @@ struct WidomTables {
   int box = -1, species = -1;
   char *dev = nullptr;
   size_t cap = 0;
-  size_t o_fm = 0, o_fs = 0, o_ft = 0, o_tab = 0, o_psite = 0, o_img = 0, o_qtab = 0, o_q = 0;
+  size_t o_fm = 0, o_fs = 0, o_ft = 0, o_tab = 0, o_psite = 0, o_img = 0, o_qtab = 0, o_q = 0, o_rmin = 0;
+  bool has_rmin = false;    // per (kind, frame site) overlap radii^2 at o_rmin
+  double rmin_max2 = 0.0;   // their largest value
   int nmols = 0, ntab = 0;
   bool split_ok = false;  // frame within the split kernels' packing limits
   int64_t ps_stride = 0;

@@ -360,6 +360,62 @@ extern "C" int cass_set_energy_table(cass_engine *e, int32_t box, int32_t res, d
   return build_energy_table(e, b);
 }
 
+// The type-pair core radii^2 of calc_rmin (create_nonbond_table.f90:285-301,
+// 398-417): rcut_low^2 unless the pair's eps > 0, then sig^2 / lambda^(1/3)
+// with lambda = (1 + sqrt(1 + U_max / eps)) / 2, the r where the LJ energy
+// reaches U_max = max(U_max_base - min_U_q, 0); min_U_q is the most negative
+// Coulomb energy the two types' charge ranges allow (at rcut_low for
+// opposite signs, at sigma otherwise).  Charge ranges per type over every
+// species' atoms (:143-152).
+extern "C" int cass_rminsq_table(cass_engine *e, int32_t box, double u_max_base, double *rminsq) {
+  if (!e || box < 0 || box >= (int32_t)e->boxes.size() || !rminsq) return CASS_ERR_ARG;
+  const cass_params &p = e->boxes[box].p;
+  const int nt = e->ntypes;
+  std::vector<double> cmin(nt, 0.0), cmax(nt, 0.0);
+  std::vector<bool> seen(nt, false);
+  for (int a = 0; a < e->natoms_total; ++a) {
+    const int t = e->atom_type[a];
+    if (t < 0) continue;
+    const double q = e->atom_charge[a];
+    cmin[t] = seen[t] ? std::min(cmin[t], q) : q;
+    cmax[t] = seen[t] ? std::max(cmax[t], q) : q;
+    seen[t] = true;
+  }
+  const double rcut_low = std::sqrt(p.rcut_lowsq);
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      const size_t k = (size_t)i * nt + j;
+      rminsq[k] = p.rcut_lowsq;
+      const double eps = e->eps[k], sig = e->sig[k];
+      if (!(eps > 0.0)) continue;
+      double min_u_q = 0.0;
+      if (p.charge_style != CASS_Q_NONE) {
+        double qp = std::min(cmin[i] * cmax[j], cmax[i] * cmin[j]);
+        if (qp <= 0.0) {
+          min_u_q = qp * kChargeFactor / rcut_low;
+        } else {
+          qp = std::min(cmin[i] * cmin[j], cmax[i] * cmax[j]);
+          min_u_q = qp * kChargeFactor / sig;
+        }
+      }
+      const double u_max = std::max(u_max_base - min_u_q, 0.0);
+      const double lambda = 0.5 * (1.0 + std::sqrt(1.0 + u_max / eps));
+      rminsq[k] = sig * sig / std::pow(lambda, 1.0 / 3.0);
+    }
+  return CASS_OK;
+}
+
+extern "C" int cass_widom_set_rminsq(cass_engine *e, int32_t box, const double *rminsq) {
+  if (!e || box < 0 || box >= (int32_t)e->boxes.size()) return CASS_ERR_ARG;
+  cass_box_state &b = e->boxes[box];
+  if (rminsq)
+    b.rminsq.assign(rminsq, rminsq + (size_t)e->ntypes * e->ntypes);
+  else
+    b.rminsq.clear();
+  ++e->gen;
+  return CASS_OK;
+}
+
 extern "C" int cass_set_nonbond_params(cass_engine *e, const double *eps, const double *sig,
                                        const double *atom_charge) {
   if (!e) return CASS_ERR_ARG;

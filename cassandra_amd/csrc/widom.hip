@@ -741,7 +741,9 @@ __device__ inline void widom_finish(const WidomArgs &w, bool active, int64_t i, 
 __global__ void __launch_bounds__(kWBlock) k_widom_overlap(RealArgs ra, const FrameMol *__restrict__ fmol,
                                                            const double4 *__restrict__ fsite,
                                                            const double *__restrict__ pos,
-                                                           const int *__restrict__ order, int *__restrict__ flag) {
+                                                           const int *__restrict__ order,
+                                                           const double *__restrict__ rmin2, double rmin_max2,
+                                                           int *__restrict__ flag) {
   const int64_t s = (int64_t)blockIdx.x * kWBlock + threadIdx.x;
   const int a = __builtin_amdgcn_readfirstlane((int)min(s / ra.npad, (int64_t)ra.ns - 1));
   const int64_t idx = s - (int64_t)a * ra.npad;
@@ -755,7 +757,10 @@ __global__ void __launch_bounds__(kWBlock) k_widom_overlap(RealArgs ra, const Fr
   dy -= ra.L[1] * rint(dy * ra.Li[1]);
   dz -= ra.L[2] * rint(dz * ra.Li[2]);
   const double rad = sqrt(wave_max(dx * dx + dy * dy + dz * dz));
-  const double rlow = sqrt(ra.rc_low2);
+  // rmin2: per (kind, frame site) radii^2 of the type-pair pre-screen, or
+  // null for rcut_low^2 everywhere
+  const double rlow = sqrt(rmin2 ? rmin_max2 : ra.rc_low2);
+  const double *rk = rmin2 ? rmin2 + (int64_t)a * ra.psite_stride : nullptr;
   bool over = false;
   for (int base = 0; base < ra.nmols; base += 64) {
     const int jt = base + (int)(threadIdx.x & 63);
@@ -782,7 +787,7 @@ __global__ void __launch_bounds__(kWBlock) k_widom_overlap(RealArgs ra, const Fr
         ex = fma(-rint(ex * ra.Li[0]), ra.L[0], ex);
         ey = fma(-rint(ey * ra.Li[1]), ra.L[1], ey);
         ez = fma(-rint(ez * ra.Li[2]), ra.L[2], ez);
-        over = over || ex * ex + ey * ey + ez * ez < ra.rc_low2;
+        over = over || ex * ex + ey * ey + ez * ez < (rk ? rk[k] : ra.rc_low2);
       }
     }
   }
@@ -1193,6 +1198,24 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
         r.w = e->prec == CASS_PREC_MIXED ? (w.q[a] * kChargeFactor) * fs[f].w : w.q[a] * fs[f].w;
         psite[(size_t)a * ps_stride + f] = r;
       }
+  // per probe kind and frame site: the overlap radius^2 of the pre-screen,
+  // max(rminsq(type pair), rcut_low^2) (check_overlap with the type-pair
+  // table, sector_routines.f90:160-233, beside the r^2 < rcut_low^2 test)
+  std::vector<double> rmin;
+  double rmin_max2 = pp.rcut_lowsq;
+  if (split && !b.rminsq.empty()) {
+    rmin.assign((size_t)ns * ps_stride, pp.rcut_lowsq);
+    for (int a = 0; a < ns; ++a) {
+      const int ta = e->atom_type[e->sp_atom_off[species] + a];
+      for (size_t f = 0; f < nfs; ++f) {
+        const int tj = ft[f];
+        if (ta < 0 || tj < 0) continue;
+        const double r2 = std::max(b.rminsq[(size_t)ta * nt + tj], pp.rcut_lowsq);
+        rmin[(size_t)a * ps_stride + f] = r2;
+        rmin_max2 = std::max(rmin_max2, r2);
+      }
+    }
+  }
   // image shifts (nx L, ny L, nz L) at (nx + 1) + 3 (ny + 1) + 9 (nz + 1)
   std::vector<double4> imgtab(27);
   for (int k = 0; k < 27; ++k)
@@ -1231,6 +1254,7 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
   T.o_img = carve(sizeof(double4) * imgtab.size());
   T.o_qtab = carve(sizeof(double4) * qtab.size());
   T.o_q = carve(sizeof(double) * kMaxWidomSites);
+  T.o_rmin = carve(sizeof(double) * rmin.size());
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   if (off > T.cap) {
     if (T.dev) HIP_TRY(e, hipFree(T.dev));
@@ -1251,6 +1275,9 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
   HIP_TRY(e, put(T.o_img, imgtab.data(), sizeof(double4) * imgtab.size()));
   HIP_TRY(e, put(T.o_qtab, qtab.data(), sizeof(double4) * qtab.size()));
   HIP_TRY(e, put(T.o_q, w.q, sizeof(double) * kMaxWidomSites));
+  HIP_TRY(e, put(T.o_rmin, rmin.data(), sizeof(double) * rmin.size()));
+  T.has_rmin = !rmin.empty();
+  T.rmin_max2 = rmin_max2;
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   T.nmols = (int)fm.size();
   T.ntab = (int)qtab.size();
@@ -1319,8 +1346,10 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   // with the pair table the CBMC column is read from it
   // (energy_routines.f90:1275); the split pipeline does that, the fused
   // kernel of the other styles does not
-  if (b.apt && !common)
-    return cass_fail(e, CASS_ERR_ARG, "the pair table needs the split Widom pipeline (common pair styles, cubic box)");
+  if ((b.apt || !b.rminsq.empty()) && !common)
+    return cass_fail(e, CASS_ERR_ARG,
+                     "the pair table and the type-pair overlap radii need the split Widom pipeline (common pair "
+                     "styles, cubic box)");
   const int nt = std::max(e->ntypes, 1);
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;
@@ -1449,7 +1478,9 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
     // hard overlaps, then the surviving sites (spatial order kept) and
     // insertions (index order)
     const unsigned rblk = (unsigned)((ns * ((n + 63) / 64 * 64) + kWBlock - 1) / kWBlock);
-    hipLaunchKernelGGL(k_widom_overlap, dim3(rblk), dim3(kWBlock), 0, e->stream, ra, d_fm, d_fs, pos, v1, d_flag);
+    const double *d_rmin = T.has_rmin ? (const double *)(T.dev + T.o_rmin) : nullptr;
+    hipLaunchKernelGGL(k_widom_overlap, dim3(rblk), dim3(kWBlock), 0, e->stream, ra, d_fm, d_fs, pos, v1, d_rmin,
+                       T.rmin_max2, d_flag);
     HIP_TRY(e, hipGetLastError());
     int *live_sites = v0;
     int *live_ins = (int *)k1;

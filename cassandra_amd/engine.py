@@ -63,6 +63,8 @@ def load_library(path=LIB_PATH):
         "cass_mol_intra_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(i32)]),
         "cass_trial_energies": (i32, [vp, i32, i32, i32, P(d), i32, i32, P(d), P(i32)]),
         "cass_set_energy_table": (i32, [vp, i32, i32, d]),
+        "cass_rminsq_table": (i32, [vp, i32, d, P(d)]),
+        "cass_widom_set_rminsq": (i32, [vp, i32, P(d)]),
         "cass_total_energy": (i32, [vp, i32, i32, P(abi.CassEnergy), P(i32)]),
         "cass_lrc": (i32, [vp, i32, P(d), P(d)]),
         "cass_mol_self_energy": (i32, [vp, i32, i32, P(d)]),
@@ -272,6 +274,25 @@ class Box:
         atompair_nrg_table_routines.f90:149-261); res=0 removes it."""
         self.eng._chk(self.lib.cass_set_energy_table(self.eng.h, self.id, int(res), float(rcut_cbmcsq_max)),
                       "cass_set_energy_table")
+
+    def rminsq_table(self, u_max_base):
+        """Type-pair core radii^2 of calc_rmin (create_nonbond_table.f90:285-301,
+        398-417), ntypes x ntypes."""
+        nt = self.eng.topo.ntypes
+        out = np.zeros(nt * nt)
+        self.eng._chk(self.lib.cass_rminsq_table(self.eng.h, self.id, float(u_max_base), _p(out)),
+                      "cass_rminsq_table")
+        return out.reshape(nt, nt)
+
+    def widom_set_rminsq(self, rminsq):
+        """The type-pair overlap pre-screen of the Widom batch (check_overlap,
+        sector_routines.f90:160-233); None turns it off."""
+        if rminsq is None:
+            ptr = None
+        else:
+            self._rminsq = np.ascontiguousarray(rminsq, dtype=np.float64).ravel()
+            ptr = _p(self._rminsq)
+        self.eng._chk(self.lib.cass_widom_set_rminsq(self.eng.h, self.id, ptr), "cass_widom_set_rminsq")
 
     def trial_energies(self, species, trial_xyz, cbmc=False, exclude=-1):
         t = np.ascontiguousarray(trial_xyz, dtype=np.float64)

@@ -158,6 +158,21 @@ MODULE cass_gpu_mod
        INTEGER(C_INT32_T), VALUE :: box, res
        REAL(C_DOUBLE), VALUE :: rcut_cbmcsq_max
      END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_rminsq_table(eng, box, u_max_base, rminsq) &
+          BIND(C, NAME='cass_rminsq_table')
+       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box
+       REAL(C_DOUBLE), VALUE :: u_max_base
+       REAL(C_DOUBLE), INTENT(OUT) :: rminsq(*)
+     END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_widom_set_rminsq(eng, box, rminsq) &
+          BIND(C, NAME='cass_widom_set_rminsq')
+       IMPORT :: C_INT32_T, C_PTR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box
+       REAL(C_DOUBLE), INTENT(IN) :: rminsq(*)
+     END FUNCTION
      INTEGER(C_INT32_T) FUNCTION cass_widom_set_template(eng, box, species, xyz, e_intra_const) &
           BIND(C, NAME='cass_widom_set_template')
        IMPORT :: C_INT32_T, C_PTR, C_DOUBLE

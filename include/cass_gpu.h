@@ -222,6 +222,20 @@ int cass_trial_energies(cass_engine *eng, int32_t box, int32_t species, int32_t 
  * res = 0 removes the table.  cass_set_nonbond_params rebuilds it. */
 int cass_set_energy_table(cass_engine *eng, int32_t box, int32_t res, double rcut_cbmcsq_max);
 
+/* The per type-pair core radii^2 of the `rcut_low` calc_rmin option
+ * (create_nonbond_table.f90:285-301, 398-417; U_max_base = 708 / MINVAL(beta)
+ * by default there, input_routines.f90:6998-7013): rminsq[ntypes*ntypes]
+ * for this box's charge style.  Host arithmetic. */
+int cass_rminsq_table(cass_engine *eng, int32_t box, double u_max_base, double *rminsq);
+
+/* The Widom core-overlap pre-screen of the cell lists (l_sectors): an
+ * insertion overlaps when a probe site lies closer to any frame site than the
+ * pair's rminsq[ntypes*ntypes] (check_overlap, sector_routines.f90:160-233,
+ * called from fragment_growth.f90:477), besides the r^2 < rcut_low^2 test.
+ * The spatial sort of the batch stands in for the sector grid.  NULL turns it
+ * off.  Split Widom pipeline only (see cass_set_energy_table). */
+int cass_widom_set_rminsq(cass_engine *eng, int32_t box, const double *rminsq);
+
 /* Same, for the trial sets of several boxes in one persistent launch:
  * set s has ntrial[s] trials in box box_of_set[s], species species_of_set[s],
  * coordinates at trial_xyz + 3*xyz_offset[s], results at nrg/overlap +

@@ -45,6 +45,8 @@ typedef struct orc_box {
   /* tabulated CBMC energies (energy_table keyword): 0 bins = off */
   int apt_res;
   double apt_step, apt_shift, apt_rcs;
+  /* type-pair core radii^2 of the cell-list overlap test; NULL = off */
+  double *rminsq;
 } orc_box;
 
 /* -------------------------------------------------- params (derived block) */
@@ -771,7 +773,7 @@ void orc_box_destroy(orc_box *b) {
   if (!b) return;
   void *ps[] = {b->mol_species, b->mol_live, b->mol_first, b->x, b->y, b->z, b->com, b->max_dcom,
                 b->hx, b->hy, b->hz, b->hsq, b->cn, b->cs, b->sn, b->cs_old, b->sn_old, b->cmol,
-                b->smol, b->cmol_save, b->smol_save};
+                b->smol, b->cmol_save, b->smol_save, b->rminsq};
   for (unsigned k = 0; k < sizeof(ps) / sizeof(ps[0]); ++k) free(ps[k]);
   free(b);
 }
@@ -931,6 +933,89 @@ int orc_total_energy(orc_box *b, cass_energy *e) {
   return 0;
 }
 
+/* rminsq_table of calc_rmin (create_nonbond_table.f90:285-301, 398-417):
+ * per type pair, rcut_low^2 unless eps > 0; charge ranges per type over all
+ * atoms (:143-152). */
+void orc_rminsq_table(const orc_box *b, double u_max_base, double *out) {
+  const cass_topology *t = &b->t;
+  const cass_params *p = &b->p;
+  int nt = t->ntypes, na = 0;
+  for (int s = 0; s < t->nspecies; ++s) na += t->species_natoms[s];
+  double cmin[64], cmax[64];
+  int seen[64] = {0};
+  for (int a = 0; a < na; ++a) {
+    int ty = t->atom_type[a];
+    if (ty < 0) continue;
+    double q = t->atom_charge[a];
+    if (!seen[ty]) {
+      cmin[ty] = q;
+      cmax[ty] = q;
+      seen[ty] = 1;
+    } else {
+      if (q < cmin[ty]) cmin[ty] = q;
+      if (q > cmax[ty]) cmax[ty] = q;
+    }
+  }
+  double rcut_low = sqrt(p->rcut_lowsq);
+  for (int i = 0; i < nt; ++i)
+    for (int j = 0; j < nt; ++j) {
+      int k = i * nt + j;
+      out[k] = p->rcut_lowsq;
+      double eps = t->eps[k], sig = t->sig[k];
+      if (!(eps > 0.0)) continue;
+      double min_u_q = 0.0;
+      if (p->charge_style != CASS_Q_NONE) {
+        double a1 = cmin[i] * cmax[j], a2 = cmax[i] * cmin[j];
+        double qp = a1 < a2 ? a1 : a2;
+        if (qp <= 0.0) {
+          min_u_q = qp * CHARGE_FACTOR / rcut_low;
+        } else {
+          a1 = cmin[i] * cmin[j];
+          a2 = cmax[i] * cmax[j];
+          qp = a1 < a2 ? a1 : a2;
+          min_u_q = qp * CHARGE_FACTOR / sig;
+        }
+      }
+      double u_max = u_max_base - min_u_q;
+      if (u_max < 0.0) u_max = 0.0;
+      double lambda = 0.5 * (1.0 + sqrt(1.0 + u_max / eps));
+      out[k] = sig * sig / pow(lambda, 1.0 / 3.0);
+    }
+}
+
+void orc_widom_set_rminsq(orc_box *b, const double *rminsq) {
+  free(b->rminsq);
+  b->rminsq = NULL;
+  if (!rminsq) return;
+  size_t n = (size_t)b->t.ntypes * b->t.ntypes;
+  b->rminsq = malloc(sizeof(double) * n);
+  memcpy(b->rminsq, rminsq, sizeof(double) * n);
+}
+
+/* check_overlap (sector_routines.f90:160-233) of every site of a placement:
+ * some box site closer than the type pair's rminsq (untyped sites:
+ * rcut_low^2).  The 27 cells around the site hold every atom within the
+ * cell length, so the scan over all sites decides the same. */
+static int check_overlap_all(const orc_box *b, int s, const double *x, const double *y, const double *z) {
+  const cass_topology *t = &b->t;
+  int oi = t->species_atom_offset[s], ni = t->species_natoms[s], nt = t->ntypes;
+  for (int ia = 0; ia < ni; ++ia) {
+    int ti = t->atom_type[oi + ia];
+    for (int m = 0; m < b->nmols; ++m) {
+      if (!b->mol_live[m]) continue;
+      int sj = b->mol_species[m], oj = t->species_atom_offset[sj];
+      for (int ja = 0; ja < t->species_natoms[sj]; ++ja) {
+        int tj = t->atom_type[oj + ja], k = b->mol_first[m] + ja;
+        double rminsq = (ti >= 0 && tj >= 0) ? b->rminsq[tj * nt + ti] : b->p.rcut_lowsq;
+        double dx, dy, dz;
+        min_image(&b->p, b->x[k] - x[ia], b->y[k] - y[ia], b->z[k] - z[ia], &dx, &dy, &dz);
+        if (dx * dx + dy * dy + dz * dz < rminsq) return 1;
+      }
+    }
+  }
+  return 0;
+}
+
 /* Widom energies of explicit placements (the per-insertion work of
  * widom_insert.f90:378-431 without CBMC placement).  out per placement:
  * {E_inter(full, combined), E_periodic_qq, E_intra_vdw, E_intra_qq, E_recip,
@@ -957,6 +1042,9 @@ void orc_widom_energies(orc_box *b, int s, int n, const double *xyz, double *out
     o[4] = (b->p.charge_style == CASS_Q_EWALD && species_has_charge(&b->t, s)) ? ewald_widom(b, s, x, y, z) : 0.0;
     int over_c = mol_inter(b, 1, s, x, y, z, com, dcom, -1, 1, &e, &dummy);
     o[5] = e;
+    /* with the cell lists, check_overlap rejects the placement first
+     * (fragment_growth.f90:477-486): both flags */
+    if (b->rminsq && check_overlap_all(b, s, x, y, z)) over = over_c = 1;
     ov[2 * i] = over;
     ov[2 * i + 1] = over_c;
   }

@@ -49,6 +49,8 @@ def _lib():
     lib.orc_total_energy.argtypes = [C.c_void_p, P(abi.CassEnergy)]
     lib.orc_widom_energies.argtypes = [C.c_void_p, C.c_int, C.c_int, P(d), P(d), P(C.c_int)]
     lib.orc_set_energy_table.argtypes = [C.c_void_p, C.c_int, d]
+    lib.orc_rminsq_table.argtypes = [C.c_void_p, d, P(d)]
+    lib.orc_widom_set_rminsq.argtypes = [C.c_void_p, P(d)]
     lib.orc_rng_init.argtypes = [P(i64), i64, i64]
     lib.orc_rng_int.restype = i64
     lib.orc_rng_int.argtypes = [P(i64)]
@@ -141,6 +143,21 @@ class OracleBox:
         """energy_table: CBMC-cutoff molecule pairs from the table of res bins
         (res=0: off)."""
         lib().orc_set_energy_table(self.h, int(res), float(rcsq_max))
+
+    def rminsq_table(self, u_max_base):
+        """rminsq_table of calc_rmin, ntypes x ntypes."""
+        nt = self.topo.ntypes
+        out = np.zeros(nt * nt)
+        lib().orc_rminsq_table(self.h, float(u_max_base), abi.ptr(out))
+        return out.reshape(nt, nt)
+
+    def widom_set_rminsq(self, rminsq):
+        """check_overlap with these type-pair radii^2 in widom_energies (None: off)."""
+        if rminsq is None:
+            lib().orc_widom_set_rminsq(self.h, None)
+        else:
+            r = np.ascontiguousarray(rminsq, dtype=np.float64).ravel()
+            lib().orc_widom_set_rminsq(self.h, abi.ptr(r))
 
     def widom_energies(self, species, placements):
         placements = np.ascontiguousarray(placements, dtype=np.float64)

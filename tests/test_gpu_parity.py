@@ -199,6 +199,46 @@ def test_trial_energies_energy_table(case):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["spce512", "lj2000"])
+def test_widom_rmin_prescreen(case):
+    """The type-pair core radii of calc_rmin (cass_rminsq_table) equal the
+    oracle's, and the Widom batch's overlap pre-screen with them
+    (cass_widom_set_rminsq, check_overlap of the cell lists) flags the same
+    insertions as the oracle, whose check_overlap is pinned to a scan of every
+    site pair; the energies of the others stay within 1e-10.  Off again, the
+    flags are those of the rcut_low test alone."""
+    from oracle import oracle as O
+    print(f"[gpu] widom rmin pre-screen {case}", flush=True)
+    system = S.with_probe(config_by_name(case), S.species_co2()).parsed()
+    eng, box = make_box(system, E.FP64)
+    box.compute_system_total_energy()
+    sp = len(system.species) - 1
+    probe = system.species[sp]
+    ref = probe.ref_coords - S.com_of(probe, probe.ref_coords)
+    rng = np.random.default_rng(41)
+    place = np.array([ref @ S.random_rotation(rng).T + (rng.random(3) - 0.5) * system.box for _ in range(300)])
+    ob = O.OracleBox(system)
+    ob.total_energy()
+    u_base = 708.0 / ob.params.beta
+    r_or = ob.rminsq_table(u_base)
+    r_gpu = box.rminsq_table(u_base)
+    assert np.allclose(r_gpu, r_or, rtol=1e-15, atol=0.0)
+    base, bov = ob.widom_energies(sp, place)
+    box.widom_set_rminsq(r_or)
+    ob.widom_set_rminsq(r_or)
+    _, _, eo = box.widom_placements(sp, place)
+    out, ov = ob.widom_energies(sp, place)
+    assert np.array_equal(eo[:, 3].astype(bool), ov[:, 0])
+    assert ov[:, 0].sum() > bov[:, 0].sum()
+    ok = ~ov[:, 0]
+    assert ok.sum() > 20
+    assert np.max(np.abs(eo[ok, 0] - out[ok, 0]) / np.maximum(np.abs(out[ok, 0]), 1.0)) < REL
+    box.widom_set_rminsq(None)
+    _, _, eo = box.widom_placements(sp, place)
+    assert np.array_equal(eo[:, 3].astype(bool), bov[:, 0])
+    eng.close()
+
+
 @pytest.mark.parametrize("case", STYLE_CASES)
 def test_widom_placements_every_style(case):
     """cass_widom_placements for a CO2 probe in every pair-style case (the
