@@ -199,6 +199,33 @@ def test_trial_energies_energy_table(case):
     eng.close()
 
 
+@pytest.mark.parametrize("case", ["spce512", "co2n2"])
+def test_regrowth_trials_exclude_self(case):
+    """Trials of a molecule already in the box, scored against every other
+    molecule (exclude_mol = itself), as regrowth and Cut_Regrow score their
+    trial positions (fragment_growth.f90:1118-1312): at the molecule's stored
+    coordinates each trial reproduces the reference's golden per-molecule
+    energy, at the CBMC and at the full cutoff."""
+    print(f"[gpu] regrowth trials {case}", flush=True)
+    g = golden_json(f"{case}_energy.json")
+    system = config_by_name(case).parsed()
+    eng, box = make_box(system)
+    xyz_all = box.download()
+    first = np.concatenate([[0], np.cumsum([system.species[s].natoms for s in box.spec])])
+    for key, cbmc in (("mol", False), ("cbmc", True)):
+        for is_, lm, ev, eq, ov in g[key][::11]:
+            m = mol_index(system, is_, lm)
+            n = system.species[is_ - 1].natoms
+            xyz = xyz_all[3 * first[m]:3 * (first[m] + n)]
+            # two trials: the stored position and the same again
+            nrg, tov = box.trial_energies(is_ - 1, np.stack([xyz, xyz]), cbmc=cbmc, exclude=m)
+            assert bool(tov[0]) == bool(ov) and bool(tov[1]) == bool(ov)
+            if not ov:
+                ref = float(ev) + float(eq)
+                assert rel(nrg[0], ref) < REL and nrg[1] == nrg[0], (key, m, nrg[0], ref)
+    eng.close()
+
+
 @pytest.mark.parametrize("case", ["spce512", "lj2000"])
 def test_widom_rmin_prescreen(case):
     """The type-pair core radii of calc_rmin (cass_rminsq_table) equal the
