@@ -493,6 +493,26 @@ extern "C" int cass_box_create(cass_engine *e, const cass_params *p, int32_t max
   return CASS_OK;
 }
 
+// A box's geometry after a volume change (move_volume.f90:374-426 scales the
+// cell, then recomputes the cell dimensions, the k vectors and the total
+// energy).  The reciprocal sums become stale until cass_ewald_setup; the
+// caller uploads the rescaled coordinates with cass_box_upload.
+extern "C" int cass_box_set_params(cass_engine *e, int32_t box, const cass_params *p) {
+  if (!e || !p || box < 0 || box >= (int32_t)e->boxes.size()) return CASS_ERR_ARG;
+  const double half = 0.5 * std::min({p->basis_length[0], p->basis_length[1], p->basis_length[2]});
+  if (p->vdw_style != CASS_VDW_NONE && p->vdw_sum_style != CASS_SUM_MINIMUM && p->rcut_vdw > half * (1 + 1e-12))
+    return cass_fail(e, CASS_ERR_CUTOFF, "vdW cutoff exceeds half the box");
+  HIP_TRY(e, hipSetDevice(e->device));
+  HIP_TRY(e, hipStreamSynchronize(e->stream));
+  cass_box_state &b = e->boxes[box];
+  if (b.apt && !(b.apt_rcsq_max >= p->rcut_cbmcsq && b.apt_rcsq_max > p->rcut_lowsq))
+    return cass_fail(e, CASS_ERR_ARG, "the pair table's range no longer covers rcut_cbmc; reset it first");
+  b.p = *p;
+  b.ewald_ready = false;
+  ++e->gen;
+  return b.apt ? build_energy_table(e, b) : CASS_OK;
+}
+
 static int box_of(cass_engine *e, int32_t box, cass_box_state **out) {
   if (!e) return CASS_ERR_ARG;
   if (box < 0 || box >= (int)e->boxes.size()) return cass_fail(e, CASS_ERR_ARG, "bad box index");

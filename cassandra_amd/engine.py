@@ -63,6 +63,7 @@ def load_library(path=LIB_PATH):
         "cass_mol_intra_energy": (i32, [vp, i32, i32, i32, P(d), P(d), P(d), P(d), P(i32)]),
         "cass_trial_energies": (i32, [vp, i32, i32, i32, P(d), i32, i32, P(d), P(i32)]),
         "cass_set_energy_table": (i32, [vp, i32, i32, d]),
+        "cass_box_set_params": (i32, [vp, i32, P(abi.CassParams)]),
         "cass_rminsq_table": (i32, [vp, i32, d, P(d)]),
         "cass_widom_set_rminsq": (i32, [vp, i32, P(d)]),
         "cass_total_energy": (i32, [vp, i32, i32, P(abi.CassEnergy), P(i32)]),
@@ -238,6 +239,17 @@ class Box:
         self.eng._chk(self.lib.cass_box_upload(self.eng.h, self.id, self.nmols, _p(self.spec, C.c_int32),
                                                _p(self.live, C.c_int32), _p(xyz), _p(com), _p(max_dcom)),
                       "cass_box_upload")
+
+    def set_geometry(self, system: S.System):
+        """cass_box_set_params from a System of the new cell (a volume move's
+        rescaled box), then the upload of its folded coordinates; Ewald boxes
+        regenerate their k vectors.  Returns the new k-vector count or None."""
+        p = derive(system)
+        self.eng._chk(self.lib.cass_box_set_params(self.eng.h, self.id, C.byref(p)), "cass_box_set_params")
+        self.system, self.params = system, p
+        spec, live, xyz = abi.box_arrays(system)
+        self.upload(spec, live, self.fold(spec, xyz))
+        return self.ewald_setup() if system.charge_style == "ewald" else None
 
     def download(self):
         out = np.zeros(3 * self.nsites)

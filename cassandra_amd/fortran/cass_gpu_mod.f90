@@ -79,6 +79,12 @@ MODULE cass_gpu_mod
        INTEGER(C_INT32_T), VALUE :: max_mols
        INTEGER(C_INT32_T), INTENT(OUT) :: box
      END FUNCTION
+     INTEGER(C_INT32_T) FUNCTION cass_box_set_params(eng, box, p) BIND(C, NAME='cass_box_set_params')
+       IMPORT :: C_INT32_T, C_PTR, cass_params_t
+       TYPE(C_PTR), VALUE :: eng
+       INTEGER(C_INT32_T), VALUE :: box
+       TYPE(cass_params_t), INTENT(IN) :: p
+     END FUNCTION
      INTEGER(C_INT32_T) FUNCTION cass_box_upload(eng, box, nmols, mol_species, live, xyz, com, max_dcom) &
           BIND(C, NAME='cass_box_upload')
        IMPORT :: C_INT32_T, C_PTR, C_DOUBLE

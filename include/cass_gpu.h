@@ -153,6 +153,14 @@ int cass_last_stage_ms(cass_engine *eng, double *ms);
 /* Adds a box able to hold `max_mols` molecules; returns its id in *box. */
 int cass_box_create(cass_engine *eng, const cass_params *p, int32_t max_mols, int32_t *box);
 
+/* Replaces a box's geometry and settings, as a volume move leaves them
+ * (move_volume.f90:374-426, move_vol_swap.f90:322-383: the cell scaled, then
+ * Compute_Cell_Dimensions).  The reciprocal sums are stale afterwards: upload
+ * the rescaled coordinates (cass_box_upload), then cass_ewald_setup
+ * regenerates the k vectors for the new cell (energy_routines.f90:2170) and
+ * cass_total_energy recomputes the box (:2648).  A pair table is rebuilt. */
+int cass_box_set_params(cass_engine *eng, int32_t box, const cass_params *p);
+
 /* Mirrors atom_list / molecule_list of a box: nmols molecules of
  * mol_species[m], live flag live[m], coordinates xyz[3*site] (sites of
  * molecule m follow those of m-1), COM com[3*m] and max_dcom[m]

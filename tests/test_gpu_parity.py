@@ -199,6 +199,48 @@ def test_trial_energies_energy_table(case):
     eng.close()
 
 
+def scaled_system(system, f):
+    """The cell scaled by f, every molecule's centre of mass scaled with it
+    and its shape kept (move_volume.f90:374-426)."""
+    import dataclasses
+    coords = []
+    for sp, c in zip(system.species, system.coords):
+        c = np.array(c, dtype=np.float64)
+        com = np.array([S.com_of(sp, m) for m in c]).reshape(len(c), 3)
+        coords.append(c + ((f - 1.0) * com)[:, None, :])
+    return dataclasses.replace(system, coords=coords, box=np.asarray(system.box, dtype=np.float64) * f)
+
+
+@pytest.mark.parametrize("case", ["spce512", "lj2000"])
+def test_volume_change_recompute(case):
+    """A volume move's recompute on an existing box (cass_box_set_params,
+    upload of the rescaled coordinates, k vectors regenerated, total energy)
+    equals a box built fresh for the new cell, to 1e-12, and the oracle, to
+    1e-10, for an expansion and a compression."""
+    from oracle import oracle as O
+    print(f"[gpu] volume change {case}", flush=True)
+    system = config_by_name(case).parsed()
+    eng, box = make_box(system)
+    box.compute_system_total_energy()
+    for f in (1.03, 0.99):
+        s2 = scaled_system(system, f)
+        nk = box.set_geometry(s2)
+        e1 = box.compute_system_total_energy()
+        eng2, box2 = make_box(s2)
+        e2 = box2.compute_system_total_energy()
+        if system.charge_style == "ewald":
+            assert nk == box2.ewald_setup()
+        ob = O.OracleBox(s2)
+        if system.charge_style == "ewald":
+            ob.ewald_setup()
+        eo = ob.total_energy()
+        for k, v in e2.items():
+            assert rel(e1[k], v) < 1e-12, (f, k, e1[k], v)
+            assert rel(e1[k], eo[k]) < REL, (f, k, e1[k], eo[k])
+        eng2.close()
+    eng.close()
+
+
 @pytest.mark.parametrize("case", ["spce512", "co2n2"])
 def test_regrowth_trials_exclude_self(case):
     """Trials of a molecule already in the box, scored against every other
