@@ -13,6 +13,7 @@ Intra_Scaling of an MCF, and ``# VDW_Style``, ``# Charge_Style``,
 from __future__ import annotations
 
 import os
+from typing import Optional
 
 import numpy as np
 
@@ -219,3 +220,142 @@ def write_case(system: System, directory: str, run_name="case", nsteps=0, with_c
     inp = os.path.join(directory, run_name + ".inp")
     write_inp(system, inp, run_name + ".out", mcfs, "start.xyz", nsteps=nsteps, with_cbmc=with_cbmc)
     return inp
+
+
+# ------------------------------------------------------------------ readers
+def _sections(path):
+    """Lines of a Cassandra input or MCF file grouped by '# Section' header,
+    with '!' comments and blank lines dropped (the reference skips both,
+    io_utilities.f90:100-330)."""
+    out, cur = {}, None
+    with open(path) as f:
+        for raw in f:
+            line = raw.split("!", 1)[0].strip()
+            if not line:
+                continue
+            if line.startswith("#"):
+                cur = line[1:].strip().split()[0]
+                out[cur] = []
+            elif line.upper() == "END":
+                cur = None
+            elif cur is not None:
+                out[cur].append(line.split())
+    return out
+
+
+def read_mcf(path: str, name: Optional[str] = None) -> Species:
+    """The sections of an MCF file the energy engine needs (Atom_Info,
+    Bond_Info, Angle_Info, Dihedral_Info, Intra_Scaling, Fragment_Info;
+    input_routines.f90:1439, 1690, 1835, 2016, 2664, 3537), numbers read as the reference's
+    String_To_Double reads them.  The reference geometry is left at zero: it
+    comes from a frame or a fragment library."""
+    sec = _sections(path)
+    D = string_to_double
+    atoms = []
+    rows = sec.get("Atom_Info", [])
+    n = int(rows[0][0]) if rows else 0
+    for r in rows[1:1 + n]:
+        style = r[5].upper() if len(r) > 5 else "NONE"
+        a = Atom(r[1], r[2], D(r[3]), D(r[4]))
+        if style == "LJ":
+            a.eps_K, a.sigma = D(r[6]), D(r[7])
+        elif style == "MIE":
+            a.eps_K, a.sigma, a.mie_n, a.mie_m = D(r[6]), D(r[7]), D(r[8]), D(r[9])
+        atoms.append(a)
+
+    def block(key):
+        rows = sec.get(key, [])
+        k = int(rows[0][0]) if rows else 0
+        return rows[1:1 + k]
+
+    bonds = [(int(r[1]) - 1, int(r[2]) - 1, D(r[4])) for r in block("Bond_Info")]
+    angles = [(int(r[1]) - 1, int(r[2]) - 1, int(r[3]) - 1, r[4], tuple(D(v) for v in r[5:]))
+              for r in block("Angle_Info")]
+    dihedrals = [(int(r[1]) - 1, int(r[2]) - 1, int(r[3]) - 1, int(r[4]) - 1, r[5], tuple(D(v) for v in r[6:]))
+                 for r in block("Dihedral_Info")]
+    sc = sec.get("Intra_Scaling", [])
+    vdw_scale = tuple(D(v) for v in sc[0][:4]) if len(sc) > 0 else (0.0, 0.0, 0.0, 1.0)
+    q_scale = tuple(D(v) for v in sc[1][:4]) if len(sc) > 1 else vdw_scale
+    frags = None
+    fr = block("Fragment_Info")
+    if fr:
+        frags = [[int(v) - 1 for v in r[2:2 + int(r[1])]] for r in fr]
+    nm = name or os.path.splitext(os.path.basename(path))[0]
+    return Species(nm, atoms, np.zeros((len(atoms), 3)), bonds, angles, dihedrals, vdw_scale, q_scale, frags)
+
+
+def read_inp(path: str) -> System:
+    """One box of a Cassandra input file: the sections that set the energy
+    engine's state (VDW_Style, Charge_Style, Mixing_Rule, Seed_Info,
+    Rcutoff_Low, Molecule_Files, Box_Info, Temperature_Info, Prob_Translation,
+    Prob_Rotation, Start_Type read_config, CBMC_Info, Rcutoff_Low;
+    input_routines.f90:425, 1015, 3723, 4074, 4389, 5103, 6002, 6979), with the
+    MCF files and the .xyz frame it names, relative to the input file."""
+    sec = _sections(path)
+    here = os.path.dirname(os.path.abspath(path))
+    D = string_to_double
+    mf = sec["Molecule_Files"]
+    species = [read_mcf(os.path.join(here, r[0])) for r in mf]
+    kw = {}
+    v = sec.get("VDW_Style", [["none"]])[0]
+    kw["vdw_style"] = {"lj": "lj", "mie": "mie"}.get(v[0].lower(), "none")
+    if kw["vdw_style"] != "none":
+        sumst = v[1].lower()
+        if sumst in ("cut_switch", "charmm"):
+            kw["vdw_sum"] = sumst
+            kw["ron_switch"], kw["roff_switch"] = D(v[2]), D(v[3])
+            kw["rcut_vdw"] = kw["roff_switch"]
+        elif sumst == "minimum_image":
+            # half the shortest face distance (input_routines.f90:521-530)
+            kw["vdw_sum"] = sumst
+            kw["rcut_vdw"] = None
+        else:
+            kw["vdw_sum"], kw["rcut_vdw"] = sumst, D(v[2])
+    q = sec.get("Charge_Style", [["none"]])[0]
+    if q[0].lower() == "coul":
+        kind = q[1].lower()
+        kw["charge_style"] = kind
+        if kind in ("ewald", "dsf", "cut"):
+            kw["rcut_coul"] = D(q[2])
+        else:
+            kw["rcut_coul"] = None  # minimum image: half the box (input_routines.f90:759-770)
+        if kind == "ewald" and len(q) > 3:
+            kw["ewald_tol"] = D(q[3])
+        if kind == "dsf" and len(q) > 3:
+            kw["alpha_dsf"] = D(q[3])
+    if "Mixing_Rule" in sec:
+        kw["mixing"] = sec["Mixing_Rule"][0][0]
+    if "Seed_Info" in sec:
+        kw["seed"] = (int(sec["Seed_Info"][0][0]), int(sec["Seed_Info"][0][1]))
+    if "Rcutoff_Low" in sec:
+        kw["rcut_low"] = D(sec["Rcutoff_Low"][0][0])
+    b = sec["Box_Info"]
+    shape = b[1][0].lower()
+    if shape == "cubic":
+        box = np.array([D(b[2][0])] * 3)
+        kw["box_shape"] = "cubic"
+    else:
+        box = np.array([D(x) for x in b[2][:3]])
+        kw["box_shape"] = "orthogonal"
+    for f in ("rcut_vdw", "rcut_coul"):
+        if f in kw and kw[f] is None:
+            kw[f] = 0.5 * float(np.min(box))
+    temperature = D(sec["Temperature_Info"][0][0])
+    if "Prob_Translation" in sec:
+        kw["prob_translate"] = D(sec["Prob_Translation"][0][0])
+        kw["max_disp"] = D(sec["Prob_Translation"][1][0])
+    if "Prob_Rotation" in sec:
+        kw["max_rot"] = D(sec["Prob_Rotation"][1][0])
+    for r in sec.get("CBMC_Info", []):
+        if r[0].lower() == "rcut_cbmc":
+            kw["rcut_cbmc"] = D(r[1])
+    st = sec["Start_Type"][0]
+    nmols = [int(x) for x in st[1:1 + len(species)]]
+    system = System(species, nmols, [np.zeros((n, sp.natoms, 3)) for sp, n in zip(species, nmols)], box,
+                    temperature, name=os.path.splitext(os.path.basename(path))[0], **kw)
+    read_xyz(os.path.join(here, st[1 + len(species)]), system)
+    # the rigid reference geometry of each species: its first molecule
+    for sp, c in zip(species, system.coords):
+        if len(c):
+            sp.ref_coords = np.array(c[0], dtype=np.float64)
+    return system
