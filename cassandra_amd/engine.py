@@ -251,6 +251,21 @@ class Box:
         self.upload(spec, live, self.fold(spec, xyz))
         return self.ewald_setup() if system.charge_style == "ewald" else None
 
+    def load_frame(self, cell, nmols, coords):
+        """One frame of a pregenerated trajectory (Read_H_frame and
+        Set_Frame_Coords, load_next_frame.f90:170-330) into this box: the
+        cell from the frame's H matrix (cubic or orthogonal), the molecules
+        folded into it, k vectors regenerated for Ewald."""
+        import dataclasses
+        cell = np.asarray(cell, dtype=np.float64)
+        if np.any(np.abs(cell - np.diag(np.diag(cell))) > 0.0):
+            raise ValueError("frames with a non-orthogonal cell are not supported")
+        L = np.diag(cell).copy()
+        shape = "cubic" if L[0] == L[1] == L[2] else "orthogonal"
+        s2 = dataclasses.replace(self.system, box=L, box_shape=shape, nmols=list(nmols),
+                                 coords=[np.asarray(c, dtype=np.float64) for c in coords])
+        return self.set_geometry(s2)
+
     def download(self):
         out = np.zeros(3 * self.nsites)
         self.eng._chk(self.lib.cass_box_download(self.eng.h, self.id, _p(out)), "cass_box_download")

@@ -359,3 +359,47 @@ def read_inp(path: str) -> System:
         if len(c):
             sp.ref_coords = np.array(c[0], dtype=np.float64)
     return system
+
+
+def write_h_frame(f, cell, nmols):
+    """One frame of a pregenerated .H file as Read_H_frame reads it
+    (load_next_frame.f90:170-212): a first line (the volume), the 3x3 cell
+    by rows, a blank line, the species count, then (species, nmols) rows."""
+    cell = np.asarray(cell, dtype=np.float64)
+    f.write(f"{abs(np.linalg.det(cell))!r}\n")
+    for r in cell:
+        f.write(" ".join(repr(float(v)) for v in r) + "\n")
+    f.write("\n")
+    live = [(k + 1, n) for k, n in enumerate(nmols)]
+    f.write(f"{len(live)}\n")
+    for k, n in live:
+        f.write(f"{k} {n}\n")
+
+
+def read_pregen_frames(h_path: str, xyz_path: str, species):
+    """Frames of a pregenerated trajectory (the .H / .xyz pair of
+    load_next_frame.f90:170-330): yields (cell, nmols, coords) per frame,
+    cell the 3x3 matrix by rows, coords per species (nmols, natoms, 3)."""
+    with open(h_path) as fh, open(xyz_path) as fx:
+        while True:
+            first = fh.readline()
+            if not first:
+                return
+            if not first.strip():
+                continue
+            # list-directed READ: correctly rounded, as float() is
+            cell = np.array([[float(v) for v in fh.readline().split()[:3]] for _ in range(3)])
+            fh.readline()
+            nsp = int(fh.readline().split()[0])
+            nmols = [0] * len(species)
+            for _ in range(nsp):
+                r = fh.readline().split()
+                nmols[int(r[0]) - 1] = int(r[1])
+            n = int(fx.readline().split()[0])
+            fx.readline()
+            xyz = np.array([[float(v) for v in fx.readline().split()[1:4]] for _ in range(n)]).reshape(n, 3)
+            coords, k = [], 0
+            for sp, nm in zip(species, nmols):
+                coords.append(xyz[k:k + nm * sp.natoms].reshape(nm, sp.natoms, 3))
+                k += nm * sp.natoms
+            yield cell, nmols, coords

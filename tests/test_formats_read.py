@@ -58,3 +58,26 @@ def test_reference_style_mcf_comments(tmp_path):
     assert sp.bonds == [(0, 1, F.string_to_double("1.540"))]
     assert sp.fragments == [[0, 1]]
     assert sp.vdw_scale == (0.0, 0.0, 0.0, 1.0)
+
+
+def test_pregen_frames_round_trip(tmp_path):
+    """A two-frame .H / .xyz trajectory (the second frame a rescaled cell)
+    reads back to the same cells, molecule counts and coordinates."""
+    system = config_by_name("spce512").parsed()
+    frames = []
+    for f in (1.0, 1.02):
+        coords = [np.asarray(c, dtype=np.float64) * f for c in system.coords]
+        frames.append((np.diag(np.asarray(system.box, dtype=np.float64) * f), list(system.nmols), coords))
+    hp, xp = tmp_path / "t.H", tmp_path / "t.xyz"
+    with open(hp, "w") as fh, open(xp, "w") as fx:
+        for cell, nm, coords in frames:
+            F.write_h_frame(fh, cell, nm)
+            rows = [f"{a.element} {float(x)!r} {float(y)!r} {float(z)!r}"
+                    for sp, c in zip(system.species, coords) for m in c for a, (x, y, z) in zip(sp.atoms, m)]
+            fx.write(f"{len(rows)}\n\n" + "\n".join(rows) + "\n")
+    got = list(F.read_pregen_frames(str(hp), str(xp), system.species))
+    assert len(got) == 2
+    for (c1, n1, x1), (c2, n2, x2) in zip(got, frames):
+        assert np.array_equal(c1, c2) and n1 == n2
+        for a, b in zip(x1, x2):
+            assert np.array_equal(a, b)

@@ -241,6 +241,41 @@ def test_volume_change_recompute(case):
     eng.close()
 
 
+def test_pregen_frames_into_one_box(tmp_path):
+    """Frames of a pregenerated trajectory (.H / .xyz, as Widom runs read
+    them) loaded one after another into one box: each frame's total energy
+    equals a box built fresh for it, and the oracle's."""
+    from cassandra_amd import formats as F
+    from oracle import oracle as O
+    print("[gpu] pregen frames", flush=True)
+    system = config_by_name("spce512").parsed()
+    hp, xp = tmp_path / "t.H", tmp_path / "t.xyz"
+    with open(hp, "w") as fh, open(xp, "w") as fx:
+        for f in (1.0, 1.02, 0.995):
+            s2 = scaled_system(system, f)
+            F.write_h_frame(fh, np.diag(np.asarray(s2.box)), s2.nmols)
+            rows = [f"{a.element} {float(x)!r} {float(y)!r} {float(z)!r}"
+                    for sp, c in zip(s2.species, s2.coords) for m in c for a, (x, y, z) in zip(sp.atoms, m)]
+            fx.write(f"{len(rows)}\n\n" + "\n".join(rows) + "\n")
+    eng, box = make_box(system)
+    n = 0
+    for cell, nmols, coords in F.read_pregen_frames(str(hp), str(xp), system.species):
+        box.load_frame(cell, nmols, coords)
+        e1 = box.compute_system_total_energy()
+        s2 = box.system
+        eng2, box2 = make_box(s2)
+        e2 = box2.compute_system_total_energy()
+        ob = O.OracleBox(s2)
+        ob.ewald_setup()
+        eo = ob.total_energy()
+        for k, v in e2.items():
+            assert rel(e1[k], v) < 1e-12 and rel(e1[k], eo[k]) < REL, (n, k)
+        eng2.close()
+        n += 1
+    assert n == 3
+    eng.close()
+
+
 @pytest.mark.parametrize("case", ["spce512", "co2n2"])
 def test_regrowth_trials_exclude_self(case):
     """Trials of a molecule already in the box, scored against every other
