@@ -403,3 +403,36 @@ def read_pregen_frames(h_path: str, xyz_path: str, species):
                 coords.append(xyz[k:k + nm * sp.natoms].reshape(nm, sp.natoms, 3))
                 k += nm * sp.natoms
             yield cell, nmols, coords
+
+
+def write_fragment_library(path: str, elements, temperatures, energies, coords):
+    """A fragment conformation library as the fragment driver writes it
+    (fragment_driver.f90:71-90): the conformation count, then per
+    conformation 'temperature energy' and one 'element x y z' row per atom."""
+    coords = np.asarray(coords, dtype=np.float64)
+    with open(path, "w") as f:
+        f.write(f"{len(coords)}\n")
+        for t, e, c in zip(temperatures, energies, coords):
+            f.write(f"{float(t)!r} {float(e)!r}\n")
+            for el, (x, y, z) in zip(elements, c):
+                f.write(f"{el} {float(x)!r} {float(y)!r} {float(z)!r}\n")
+
+
+def read_fragment_library(path: str, natoms: int):
+    """A fragment conformation library as Get_Fragment_Coords reads it
+    (input_routines.f90:3466-3480, list-directed READ): returns
+    (temperatures, energies, coords[nconfig, natoms, 3])."""
+    with open(path) as f:
+        tokens = f.read().split()
+    k = 0
+    n = int(tokens[k])
+    k += 1
+    temps, nrg = np.zeros(n), np.zeros(n)
+    xyz = np.zeros((n, natoms, 3))
+    for c in range(n):
+        temps[c], nrg[c] = float(tokens[k]), float(tokens[k + 1])
+        k += 2
+        for a in range(natoms):
+            xyz[c, a] = [float(v) for v in tokens[k + 1:k + 4]]
+            k += 4
+    return temps, nrg, xyz

@@ -81,3 +81,15 @@ def test_pregen_frames_round_trip(tmp_path):
         assert np.array_equal(c1, c2) and n1 == n2
         for a, b in zip(x1, x2):
             assert np.array_equal(a, b)
+
+
+def test_fragment_library_round_trip(tmp_path):
+    """A conformation library (fragment_driver.f90:71-90 writes it,
+    input_routines.f90:3466-3480 reads it) reads back to the same doubles."""
+    rng = np.random.default_rng(5)
+    xyz = rng.normal(size=(7, 5, 3)) * 1.5
+    temps, nrg = np.full(7, 298.0), rng.normal(size=7) * 100.0
+    p = tmp_path / "frag.dat"
+    F.write_fragment_library(str(p), ["C", "H", "H", "H", "H"], temps, nrg, xyz)
+    t2, e2, x2 = F.read_fragment_library(str(p), 5)
+    assert np.array_equal(t2, temps) and np.array_equal(e2, nrg) and np.array_equal(x2, xyz)
