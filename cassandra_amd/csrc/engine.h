@@ -59,6 +59,9 @@ struct WidomTables {
   size_t o_fm = 0, o_fs = 0, o_ft = 0, o_tab = 0, o_psite = 0, o_img = 0, o_qtab = 0, o_q = 0, o_rmin = 0;
   bool has_rmin = false;    // per (kind, frame site) overlap radii^2 at o_rmin
   double rmin_max2 = 0.0;   // their largest value
+  size_t o_rminf = 0;       // fused path: per (probe site, frame type) radii^2
+  bool has_rminf = false;
+  double rminf_max2 = 0.0;
   int nmols = 0, ntab = 0;
   bool split_ok = false;  // frame within the split kernels' packing limits
   int64_t ps_stride = 0;

@@ -48,6 +48,9 @@ struct WidomFrame {
 struct WidomPairTab {
   const double4 *__restrict__ p;
   int ntypes;
+  // overlap radii^2 per (probe site, frame type), max(rminsq, rcut_low^2),
+  // of the type-pair pre-screen; null: rcut_low^2 everywhere
+  const double *__restrict__ rmin2;
 };
 
 // Centre of insertion i in fractional coordinates of the box, in [0, 1).
@@ -307,7 +310,7 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
       const bool near = live && (w.cutoff_only ? r2 <= w.rc_pair2 : true);
       if (!lanes_any(near)) continue;
       if (near) {
-        if (r2 < cd.rc_low2) over = true;
+        if (r2 < ((pt.rmin2 && tj >= 0) ? pt.rmin2[a * pt.ntypes + tj] : cd.rc_low2)) over = true;
         bool gv, gq, gvc, gqc;
         pair_cutoff(cd, false, r2, gv, gq);
         // the CBMC pass's own tests (Check_AtomPair_Cutoff with cbmc_flag:
@@ -1185,6 +1188,16 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
       tab2[(size_t)a * nt + t] = make_double4(4.0 * eps, sig * sig, shift, 0.0);
     }
 
+  // the fused path's overlap radii^2 per (probe site, frame type)
+  std::vector<double> rminf;
+  if (!b.rminsq.empty()) {
+    rminf.assign((size_t)ns * nt, pp.rcut_lowsq);
+    for (int a = 0; a < ns; ++a)
+      for (int t = 0; t < e->ntypes; ++t)
+        if (w.type[a] >= 0)
+          rminf[(size_t)a * nt + t] = std::max(b.rminsq[(size_t)w.type[a] * e->ntypes + t], pp.rcut_lowsq);
+  }
+
   // per probe kind, one record per frame site: the LJ row entry and the
   // charge product q_a q_j (times charge_factor in mixed precision, the
   // factor the pair function of that mode leaves out)
@@ -1255,6 +1268,7 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
   T.o_qtab = carve(sizeof(double4) * qtab.size());
   T.o_q = carve(sizeof(double) * kMaxWidomSites);
   T.o_rmin = carve(sizeof(double) * rmin.size());
+  T.o_rminf = carve(sizeof(double) * rminf.size());
   HIP_TRY(e, hipStreamSynchronize(e->stream));
   if (off > T.cap) {
     if (T.dev) HIP_TRY(e, hipFree(T.dev));
@@ -1276,6 +1290,9 @@ static int widom_tables(cass_engine *e, const cass_box_state &b, int box, int sp
   HIP_TRY(e, put(T.o_qtab, qtab.data(), sizeof(double4) * qtab.size()));
   HIP_TRY(e, put(T.o_q, w.q, sizeof(double) * kMaxWidomSites));
   HIP_TRY(e, put(T.o_rmin, rmin.data(), sizeof(double) * rmin.size()));
+  HIP_TRY(e, put(T.o_rminf, rminf.data(), sizeof(double) * rminf.size()));
+  T.has_rminf = !rminf.empty();
+  T.rminf_max2 = rminf.empty() ? 0.0 : *std::max_element(rminf.begin(), rminf.end());
   T.has_rmin = !rmin.empty();
   T.rmin_max2 = rmin_max2;
   HIP_TRY(e, hipStreamSynchronize(e->stream));
@@ -1346,10 +1363,8 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   // with the pair table the CBMC column is read from it
   // (energy_routines.f90:1275); the split pipeline does that, the fused
   // kernel of the other styles does not
-  if ((b.apt || !b.rminsq.empty()) && !common)
-    return cass_fail(e, CASS_ERR_ARG,
-                     "the pair table and the type-pair overlap radii need the split Widom pipeline (common pair "
-                     "styles, cubic box)");
+  if (b.apt && !common)
+    return cass_fail(e, CASS_ERR_ARG, "the pair table needs the split Widom pipeline (common pair styles, cubic box)");
   const int nt = std::max(e->ntypes, 1);
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;
@@ -1510,7 +1525,10 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
                        d_eout);
     HIP_TRY(e, hipGetLastError());
   } else {
-    WidomPairTab pt{d_tab, nt};
+    // every pair inside an overlap radius must reach the pair code
+    if (T.has_rminf && T.rminf_max2 > w.rc_pair2)
+      return cass_fail(e, CASS_ERR_ARG, "type-pair overlap radius beyond the pair cutoff");
+    WidomPairTab pt{d_tab, nt, T.has_rminf ? (const double *)(T.dev + T.o_rminf) : nullptr};
     hipLaunchKernelGGL(k_widom_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, e->stream, w, bv, g[0], g[1],
                        g[2], k0, v0);
     HIP_TRY(e, hipGetLastError());

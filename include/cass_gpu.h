@@ -241,7 +241,7 @@ int cass_rminsq_table(cass_engine *eng, int32_t box, double u_max_base, double *
  * pair's rminsq[ntypes*ntypes] (check_overlap, sector_routines.f90:160-233,
  * called from fragment_growth.f90:477), besides the r^2 < rcut_low^2 test.
  * The spatial sort of the batch stands in for the sector grid.  NULL turns it
- * off.  Split Widom pipeline only (see cass_set_energy_table). */
+ * off.  Radii beyond the pair cutoff are refused. */
 int cass_widom_set_rminsq(cass_engine *eng, int32_t box, const double *rminsq);
 
 /* Same, for the trial sets of several boxes in one persistent launch:

@@ -303,12 +303,12 @@ def test_regrowth_trials_exclude_self(case):
     eng.close()
 
 
-@pytest.mark.parametrize("case", ["spce512", "lj2000"])
+@pytest.mark.parametrize("case", ["spce512", "lj2000", "sty_switch_cut", "sty_mie_cut_shift"])
 def test_widom_rmin_prescreen(case):
     """The type-pair core radii of calc_rmin (cass_rminsq_table) equal the
     oracle's, and the Widom batch's overlap pre-screen with them
-    (cass_widom_set_rminsq, check_overlap of the cell lists) flags the same
-    insertions as the oracle, whose check_overlap is pinned to a scan of every
+    (cass_widom_set_rminsq, check_overlap of the cell lists; split pipeline
+    and fused kernel) flags the same insertions as the oracle, whose check_overlap is pinned to a scan of every
     site pair; the energies of the others stay within 1e-10.  Off again, the
     flags are those of the rcut_low test alone."""
     from oracle import oracle as O
