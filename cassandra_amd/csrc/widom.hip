@@ -338,7 +338,21 @@ __device__ inline void molecule_pairs(const FrameMol &m, const WidomFrame &f, co
         }
         if (gv) ev += (double)e_v;
         if (gq) eq += (double)e_q;
-        if (gvc || gqc) ecb += (gvc ? (double)e_v : 0.0) + (gqc ? (double)e_q : 0.0);
+        if (b.apt_res > 0) {
+          // the CBMC term from the pair table (Estimate_MoleculePair_Energy,
+          // energy_routines.f90:1528-1587), with the lookup's own roundings
+          const double r2s =
+              __dsub_rn(__dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz)), b.apt_shift);
+          if (r2s < b.apt_rcs && r2s >= b.apt_step) {
+            const int res = b.apt_res, nt = pt.ntypes;
+            const int bin = max(0, min(res, (int)(r2s / b.apt_step)) - 1);
+            const double v = (tj >= 0 && w.type[a] >= 0) ? b.apt[(size_t)(tj * nt + w.type[a]) * res + bin] : 0.0;
+            const double f2 = b.apt[(size_t)nt * nt * res + bin];
+            ecb = __dadd_rn(ecb, __dadd_rn(v, __dmul_rn(f2, __dmul_rn(__dmul_rn(w.q[a], kChargeFactor), st.w))));
+          }
+        } else if (gvc || gqc) {
+          ecb += (gvc ? (double)e_v : 0.0) + (gqc ? (double)e_q : 0.0);
+        }
       }
     }
   }
@@ -1360,11 +1374,7 @@ int cass_run_widom(cass_engine *e, int32_t box, int32_t species, int64_t n_ins, 
   }
   const WidomTables &T = e->wtab;
   common = common && T.split_ok;
-  // with the pair table the CBMC column is read from it
-  // (energy_routines.f90:1275); the split pipeline does that, the fused
-  // kernel of the other styles does not
-  if (b.apt && !common)
-    return cass_fail(e, CASS_ERR_ARG, "the pair table needs the split Widom pipeline (common pair styles, cubic box)");
+
   const int nt = std::max(e->ntypes, 1);
   const size_t n = (size_t)n_ins;
   const size_t nslot = common ? n * ns : n;

@@ -224,9 +224,7 @@ int cass_trial_energies(cass_engine *eng, int32_t box, int32_t species, int32_t 
  * Compute_Molecule_Nonbond_Inter_Energy does (:1180-1182): vdW and q-q of
  * the table together in e_vdw / nrg, the overlap test r^2 < rcut_low^2.
  * The Widom batch reads its CBMC column from the table too
- * (energy_routines.f90:1275) for the pair styles of its split pipeline (LJ
- * cut / cut_tail / cut_shift with Ewald, cut Coulomb or no charges, cubic
- * box) and returns CASS_ERR_ARG for the others while a table is set.
+ * (energy_routines.f90:1275).
  * res = 0 removes the table.  cass_set_nonbond_params rebuilds it. */
 int cass_set_energy_table(cass_engine *eng, int32_t box, int32_t res, double rcut_cbmcsq_max);
 

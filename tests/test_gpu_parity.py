@@ -144,9 +144,6 @@ STYLE_CASES = ["sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut", "sty_shiftfor
 
 TABLE_CASES = ["spce512", "sty_cut_ewald", "sty_shift_dsf", "sty_switch_cut", "sty_shiftforce_dsfa",
                "sty_charmm_ewald", "sty_ortho_ewald", "sty_mie_cut_shift", "sty_minimum_minimum"]
-# cases the split Widom pipeline serves (LJ cut / cut_tail / cut_shift with
-# Ewald, cut Coulomb or no charges, cubic box)
-SPLIT_CASES = {"spce512", "sty_cut_ewald"}
 
 
 @pytest.mark.parametrize("case", TABLE_CASES)
@@ -155,8 +152,8 @@ def test_trial_energies_energy_table(case):
     the energy_table keyword) against the oracle's Estimate_MoleculePair_Energy
     at a coarse and the default table: within 1e-12, identical overlap flags,
     in every precision mode (the lookup is FP64 in all).  The Widom batch's
-    CBMC column reads the table too (split pipeline; the fused kernel of the
-    other styles refuses a table).  Removing the table gives the direct CBMC
+    CBMC column reads the table too, in the split pipeline and in the fused
+    kernel of the other styles.  Removing the table gives the direct CBMC
     energies back."""
     from oracle import oracle as O
     print(f"[gpu] energy table {case}", flush=True)
@@ -182,15 +179,11 @@ def test_trial_energies_energy_table(case):
             assert ok.sum() > 20
             assert np.max(np.abs(nrg[ok] - out[ok, 5]) / np.maximum(np.abs(out[ok, 5]), 1.0)) < 1e-12
         eng.set_precision(E.FP64)
-        if case in SPLIT_CASES:
-            _, _, eo = box.widom_placements(sp, place)
-            assert np.array_equal(eo[:, 3].astype(bool), ov[:, 0])
-            ok = ~ov[:, 0]
-            assert np.max(np.abs(eo[ok, 1] - out[ok, 5]) / np.maximum(np.abs(out[ok, 5]), 1.0)) < 1e-12
-            assert np.max(np.abs(eo[ok, 0] - direct[ok, 0]) / np.maximum(np.abs(direct[ok, 0]), 1.0)) < REL
-        else:
-            with pytest.raises(RuntimeError):
-                box.widom_placements(sp, place[:4])
+        _, _, eo = box.widom_placements(sp, place)
+        assert np.array_equal(eo[:, 3].astype(bool), ov[:, 0])
+        ok = ~ov[:, 0]
+        assert np.max(np.abs(eo[ok, 1] - out[ok, 5]) / np.maximum(np.abs(out[ok, 5]), 1.0)) < 1e-12
+        assert np.max(np.abs(eo[ok, 0] - direct[ok, 0]) / np.maximum(np.abs(direct[ok, 0]), 1.0)) < REL
     box.set_energy_table(0)
     nrg, tov = box.trial_energies(sp, place, cbmc=True)
     assert np.array_equal(tov, dov[:, 1])
